@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Throughput of the device-resident hydro force path on one MI355X.
+
+Workload (default): the 3D dam break -- water column 1.0 x 0.5 x 2.0 m in a
+4.0 x 0.5 x 3.0 m tank with 3 wall layers, dx = 0.0126 m (502,440 fluid +
+562,938 wall = 1,065,378 particles), h = 1.3 dx, alpha = 0.1,
+c0 = 10 sqrt(2 g H), CFL 0.25 -- stepped on the GPU. A step is one full
+time step: kick-drift, cell-list rebuild by counting sort, neighbour pass,
+force pass, gravity/final kick and the global dt reduction, all in HBM.
+
+value = neighbour pair interactions evaluated in the timed steps / wall time
+of those steps. The roofline entry is for the force kernel (k_force), from
+HIP events recorded on the simulation's stream around every timed launch.
+The CPU baseline times the FP64 oracle's force pass (the reference CPU
+functor, src/force.h:31-60, over cell-list candidates, OpenMP) on the same
+developed state, on the host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_PAIR = {3: 73, 2: 63}       # algorithmic flops per pair interaction
+PEAK_FP32_TFLOPS = 157.3             # MI355X published FP32 peak
+PEAK_HBM_GBS = 8000.0
+
+
+def cpu_baseline(sim, params, budget_s, threads):
+    import numpy as np
+    import oracle as O
+    out = sim.download()
+    st0 = sim.initial_state
+    s = O.State(sim.dim, sim.n)
+    s.x[:] = out["pos"]
+    s.v[:] = out["vel"]
+    s.dens[:] = out["dens"]
+    s.mass[:] = st0["mass"]
+    s.h[:] = st0["h"]
+    s.type[:] = st0["type"]
+    op = O.sim_params(sim.dim, alpha=params.alpha, beta=params.beta, eps=params.eps, c_cfl=params.c_cfl,
+                      vsig_mode=params.vsig_mode, rho0=params.rho0, c0=params.c0, gamma=params.gamma, g=params.g,
+                      nu=params.nu, rho_floor=params.rho_floor)
+    s.eos(op)
+    reps, pairs, t = 0, 0, 0.0
+    while t < budget_s or reps == 0:
+        t0 = time.perf_counter()
+        pairs += s.force(op, threads)
+        t += time.perf_counter() - t0
+        reps += 1
+        if t > 0.5 * budget_s and t / reps * (reps + 1) > 1.5 * budget_s:
+            break
+    return {"value": pairs / t, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
+            "sample": f"{reps} FP64 force pass(es) of the oracle (src/force.h:31-60 over 27-cell candidates) "
+                      f"on the developed {sim.n}-particle state, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed", "fp64"])
+    ap.add_argument("--dim", type=int, default=3, choices=[2, 3])
+    ap.add_argument("--dx", type=float, default=None)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 skips it)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    import weaklycompressiblesph_amd as W
+
+    dim = args.dim
+    dx = args.dx or (0.0126 if dim == 3 else 0.02)
+    params = W.dam_break_params(dim)
+    desc = W.dam_break_desc(dim, dx)
+    st = W.dam_break_state(desc, params)
+    n = len(st["dens"])
+    print(f"# {dim}D dam break dx={dx}: {st['n_fluid']} fluid + {st['n_wall']} wall = {n} particles, "
+          f"precision {args.precision}", file=sys.stderr, flush=True)
+    sim = W.Simulation.from_state(params, st, precision=args.precision)
+    sim.initial_state = st
+    sim.init()
+    t0 = time.perf_counter()
+    sim.step(args.warmup)
+    sim.sync()
+    print(f"# warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+
+    s0 = sim.stats()
+    sim.set_timing(True)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(args.steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    s1 = sim.stats()
+    tm = sim.timing()
+    sim.set_timing(False)
+    wall = t1 - t0
+    pairs = s1["pairs_total"] - s0["pairs_total"]
+    K = args.steps
+    force_ms = tm["force"] / K
+    pairs_launch = pairs / K
+    achieved = FLOP_PER_PAIR[dim] * pairs_launch / (force_ms * 1e-3) / 1e12
+    dtype = {"fp32": "fp32", "mixed": "fp16x2/fp32", "fp64": "fp64"}[args.precision]
+    line = {
+        "metric": "pair-interactions/s",
+        "value": pairs / wall,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (dam-break lattice initial condition, developed by the warmup steps)",
+        "config": {"workload": f"{dim}D dam break, dx={dx} m, {n} particles ({st['n_fluid']} fluid), "
+                               f"{args.precision} force path, full device-resident steps",
+                   "particles": n, "dx": dx, "precision": args.precision, "step_window":
+                       f"steps {args.warmup + 1}-{args.warmup + K}"},
+        "steps_per_s": K / wall,
+        "pairs_per_step": pairs_launch,
+        "force_kernel_pairs_per_s": pairs_launch / (force_ms * 1e-3),
+        "stage_ms_per_step": {k: tm[k] / K for k in ("kick_bin_reorder", "neighbour", "force", "final", "step")},
+        "sim_time_s": s1["time"],
+        "escaped": s1["escaped"],
+        "overflow": s1["overflow"],
+        "roofline": {"bound": "valu", "kernel": "k_force", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "flop_per_pair": FLOP_PER_PAIR[dim], "avg_launch_ms": force_ms},
+    }
+    if rank == 0 and args.cpu_seconds > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(f"# cpu baseline on {threads} threads", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(sim, params, args.cpu_seconds, threads)
+    sim.close()
+    if rank == 0:
+        print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,185 @@
+/*
+ * wcsph_abi.h -- C-ABI of the MI355X (gfx950) hydro force path for a weakly
+ * compressible SPH solver.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The FDPS multiwalk drop-in. These replace the reference's CUDA backend
+ *     GPU_WC::HydrDispatchKernel / GPU_WC::HydrRetrieveKernel
+ *     (src/prototype.h:7-10, defined at src/kernel.cu:289-381 and :383-409).
+ *     Records are passed in the reference's own binary layouts
+ *     (src/class.h:63-67 RESULT::Hydro, :167-176 EPI::Hydro,
+ *     :212-223 EPJ::Hydro) so FDPS's buffers are handed over without copies.
+ *     include/wcsph_fdps.hpp wraps them in the exact reference signatures.
+ *
+ *  2. A device-resident simulation (wcsph_sim_*): cell list by counting sort,
+ *     neighbour pass, force pass, FP64 kick-drift-kick and the global time
+ *     step, all kept in HBM. These replace the whole per-step loop of
+ *     src/main.cpp:89-133 (FDPS tree + dispatch + retrieve + the OpenMP
+ *     integrator loops of src/class.h:146-163 and src/integral.h:2-12).
+ *
+ * Conventions: every function returns 0 on success and a nonzero WCSPH_E*
+ * code on failure; wcsph_last_error() returns a message for the calling
+ * thread's last failure. Nothing throws across this boundary and nothing
+ * reads stdin (the reference asks for a device id on stdin,
+ * src/kernel.cu:295-297; here it is wcsph_params.device).
+ */
+#ifndef WCSPH_ABI_H
+#define WCSPH_ABI_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WCSPH_OK 0
+#define WCSPH_EINVAL 1      /* bad argument */
+#define WCSPH_EHIP 2        /* HIP runtime failure */
+#define WCSPH_ESTATE 3      /* call out of order (e.g. retrieve without dispatch) */
+#define WCSPH_ENOMEM 4      /* allocation failed or over a stated budget */
+#define WCSPH_EOVERFLOW 5   /* neighbour list or cell grid capacity exceeded */
+
+/* ---------------- FDPS boundary records (64-bit FDPS build) ----------------
+ * 3D sizes: EPI 96 B, EPJ 112 B, RESULT 40 B; 2D: 80 / 96 / 32 B. */
+typedef struct { double pos[3]; double vel[3]; double smth, dens, pres, snds, visc; int64_t tag; } wcsph_epi_hydro_3d;
+typedef struct { double pos[2]; double vel[2]; double smth, dens, pres, snds, visc; int64_t tag; } wcsph_epi_hydro_2d;
+typedef struct { double pos[3]; double vel[3]; double dens, mass, smth, pres, snds, visc; int64_t tag; int32_t type; int32_t pad_; } wcsph_epj_hydro_3d;
+typedef struct { double pos[2]; double vel[2]; double dens, mass, smth, pres, snds, visc; int64_t tag; int32_t type; int32_t pad_; } wcsph_epj_hydro_2d;
+typedef struct { double acc[3]; double dt, div_v; } wcsph_result_hydro_3d;
+typedef struct { double acc[2]; double dt, div_v; } wcsph_result_hydro_2d;
+
+/* Physics and numerics parameters (one POD across the ABI). */
+typedef struct {
+    int32_t dim;        /* 2 or 3 */
+    int32_t vsig_mode;  /* 0: v_sig = max(v_sig, mu) (src/force.h:47, gives v_sig = 0);
+                           1: v_sig = max(v_sig, -mu) (src/kernel.cu:263) */
+    double alpha;       /* linear artificial viscosity (src/force.h:32: 0.01) */
+    double beta;        /* quadratic artificial viscosity (src/force.h:46: none) */
+    double eps;         /* regulariser in mu and the viscous term (src/force.h:45,53: 0.01) */
+    double c_cfl;       /* Courant factor (src/param.h:5: 0.3) */
+    double rho0;        /* Tait reference density (src/init/Dam2.h:39) */
+    double c0;          /* Tait reference sound speed (src/init/Dam2.h:42) */
+    double gamma;       /* Tait exponent (src/init/Dam2.h:38) */
+    double g;           /* gravity along the last axis (src/init/Dam2.h:106) */
+    double nu;          /* kinematic viscosity for the simulation (src/EoS.h:38) */
+    double rho_floor;   /* density floor after each step (src/init/Dam2.h:111; 0 = off) */
+    int32_t device;     /* HIP device ordinal */
+    int32_t reserved_;
+} wcsph_params;
+
+/* Fill p with the reference CPU functor's constants (src/force.h:32,45-47,
+ * src/param.h:5, src/init/Dam2.h:38-43,106, src/EoS.h:38). */
+int wcsph_params_reference(int32_t dim, wcsph_params* p);
+const char* wcsph_last_error(void);
+/* Library version string, for logs. */
+const char* wcsph_version(void);
+
+/* ---------------- 1. FDPS multiwalk drop-in ---------------- */
+/* Replaces the lazy first-call set-up of src/kernel.cu:290-305. */
+int wcsph_init(const wcsph_params* p);
+int wcsph_finalize(void);
+
+/* GPU_WC::HydrDispatchKernel (src/kernel.cu:289-381). Copies the walks into
+ * pinned staging (FDPS reuses its buffers after the call), starts the
+ * upload and the force kernel on the tag's stream, and returns with the work
+ * in flight. tag in [0, 8). At most one outstanding batch per tag. */
+int wcsph_hydro_dispatch_3d(int32_t tag, int32_t n_walk, const wcsph_epi_hydro_3d* const* epi,
+                            const int32_t* n_epi, const wcsph_epj_hydro_3d* const* epj,
+                            const int32_t* n_epj);
+int wcsph_hydro_dispatch_2d(int32_t tag, int32_t n_walk, const wcsph_epi_hydro_2d* const* epi,
+                            const int32_t* n_epi, const wcsph_epj_hydro_2d* const* epj,
+                            const int32_t* n_epj);
+/* GPU_WC::HydrRetrieveKernel (src/kernel.cu:383-409). Waits for the tag's
+ * batch and assigns acc, dt and div_v of every i of every walk. */
+int wcsph_hydro_retrieve_3d(int32_t tag, int32_t n_walk, const int32_t* ni, wcsph_result_hydro_3d** force);
+int wcsph_hydro_retrieve_2d(int32_t tag, int32_t n_walk, const int32_t* ni, wcsph_result_hydro_2d** force);
+/* Device time (ms) of the last force kernel of a tag, from HIP events. */
+int wcsph_hydro_last_kernel_ms(int32_t tag, float* ms);
+
+/* ---------------- 2. device-resident simulation ---------------- */
+#define WCSPH_PREC_FP32 0   /* FP32 pair arithmetic and accumulation */
+#define WCSPH_PREC_MIXED 1  /* h-scaled FP16 pair differences, packed FP16, FP32 accumulation */
+#define WCSPH_PREC_FP64 2   /* FP64 everywhere (the GPU oracle) */
+
+typedef struct wcsph_sim wcsph_sim;
+
+typedef struct {
+    int32_t precision;      /* WCSPH_PREC_* */
+    int32_t max_neighbours; /* neighbour list capacity per particle (0: 128) */
+    double box_lo[3];       /* cell-grid domain; particles outside are clamped */
+    double box_hi[3];       /*   into the edge cells and counted as escaped */
+} wcsph_sim_config;
+
+typedef struct {
+    double time, dt;
+    int64_t step;
+    int64_t n;
+    int64_t pairs_last;     /* neighbour pairs (i != j, r < hi+hj) of the last force pass */
+    int64_t pairs_total;    /* summed over all force passes */
+    int64_t escaped;        /* particles clamped into the grid edge at the last binning */
+    int32_t max_count;      /* largest neighbour count of the last pass */
+    int32_t overflow;       /* nonzero if a neighbour list was truncated */
+} wcsph_sim_stats;
+
+/* Dam-break initial condition on a square/cubic lattice (the geometry of the
+ * dam-break configurations: column col_w x depth x col_h at x in [0,col_w],
+ * tank tank_l x depth x tank_h, nwall wall layers on the sides and bottom,
+ * hydrostatic density, h = hfac*dx, m = rho*dx^D). 2D ignores depth. */
+typedef struct {
+    int32_t dim;
+    int32_t nwall;
+    double dx, col_w, col_h, depth, tank_l, tank_h, hfac;
+} wcsph_dam_break_desc;
+
+/* Number of particles the description generates (fluid and wall). */
+int wcsph_dam_break_count(const wcsph_dam_break_desc* d, int64_t* n_fluid, int64_t* n_wall);
+/* Host-side generation into caller SoA arrays: pos/vel [dim][n], others [n]. */
+int wcsph_dam_break_fill(const wcsph_dam_break_desc* d, const wcsph_params* p, double* pos,
+                         double* vel, double* dens, double* mass, double* h, int32_t* type);
+/* Device bytes a simulation of n particles needs (for admission control). */
+int64_t wcsph_sim_bytes(int32_t dim, int64_t n, const wcsph_sim_config* cfg);
+
+int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t n, wcsph_sim** out);
+int wcsph_sim_destroy(wcsph_sim* s);
+/* Upload a state (SoA, host): pos/vel [dim][n], dens/mass/h [n], type [n]
+ * (0 fluid, 1 wall = FREEZE of src/class.h:33-36). acc, div_v and the
+ * per-particle dt are then produced by wcsph_sim_init. */
+int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const double* dens,
+                     const double* mass, const double* h, const int32_t* type);
+/* Pressure, first force pass and gravity (src/main.cpp:69-79, no filter). */
+int wcsph_sim_init(wcsph_sim* s);
+/* nsteps steps of src/main.cpp:89-133 (no Shepard filter), queued on the
+ * simulation's stream; returns without waiting. */
+int wcsph_sim_step(wcsph_sim* s, int32_t nsteps);
+/* Re-bin and run one force pass on the current state (no integration):
+ * acc (hydro only, no gravity), div_v and dt_i of the current positions. */
+int wcsph_sim_force(wcsph_sim* s);
+int wcsph_sim_sync(wcsph_sim* s);
+int wcsph_sim_get_stats(wcsph_sim* s, wcsph_sim_stats* st);
+/* Download in the original particle order (index given at upload). Any
+ * pointer may be NULL. acc is hydro + gravity after a step, hydro only after
+ * wcsph_sim_force. */
+int wcsph_sim_download(wcsph_sim* s, double* pos, double* vel, double* dens, double* acc,
+                       double* div_v, double* dt_i);
+/* Neighbour counts (and lists of original indices, ascending, if lists is
+ * not NULL) of the last force pass, in original order; lists is [n][cap]. */
+int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t cap);
+/* FP64 reductions on the device: out[0] mass, out[1..3] momentum,
+ * out[4] kinetic, out[5] potential, out[6] Tait internal energy,
+ * out[7] dam front (max x of fluid with vertical coordinate < zfront),
+ * out[8] column height (max vertical coordinate of fluid with x < xcol). */
+int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double* out);
+/* Device time (ms) of the kernels of the last wcsph_sim_force call:
+ * out[0] binning, out[1] neighbour pass, out[2] force pass. */
+int wcsph_sim_last_force_ms(wcsph_sim* s, float* out);
+/* Per-step HIP-event timing on the simulation's stream. While enabled, each
+ * step records events around its stages; wcsph_sim_timing waits, returns the
+ * summed device times (ms) since enabling -- out[0] kick-drift + binning +
+ * reorder, out[1] neighbour pass, out[2] force pass, out[3] final stage,
+ * out[4] whole steps -- with *nsteps timed, and restarts the sums. */
+int wcsph_sim_set_timing(wcsph_sim* s, int32_t enable);
+int wcsph_sim_timing(wcsph_sim* s, double* out, int64_t* nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+"""GPU parity of the device-resident step (cell list by counting sort,
+neighbour pass, force pass, FP64 kick-drift-kick, global dt) against the
+FP64 oracle.
+
+Tolerances: neighbour sets bit-exact; one force pass normwise relative
+<= 1e-5 (FP32 path) and <= 1e-12 (FP64 path) on acc and div_v; multi-step
+trajectories as stated per test.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def developed(O, W, dim, dx, steps, **geom):
+    p = W.dam_break_params(dim)
+    s = O.State.dam_break(dim, dx, c0=p.c0, g=p.g, **geom)
+    op = O.sim_params(dim, alpha=p.alpha, c_cfl=p.c_cfl, c0=p.c0, g=p.g, nu=p.nu)
+    s.init(op, 8)
+    for _ in range(steps):
+        s.step(op, 8)
+    return s, p, op
+
+
+def state_dict(s):
+    return dict(pos=s.x, vel=s.v, dens=s.dens, mass=s.mass, h=s.h, type=s.type)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+SMALL3 = dict(tank_l=2.0, col_h=1.0, tank_h=1.5)
+
+
+@pytest.mark.parametrize("dim,dx,geom", [(2, 0.02, {}), (3, 0.04, SMALL3)])
+def test_neighbour_sets_bit_exact(O, W, dim, dx, geom):
+    s, p, _ = developed(O, W, dim, dx, 25, **geom)
+    # jitter so many pairs sit near the support boundary
+    rng = np.random.default_rng(11)
+    s.x[:] += rng.uniform(-0.3, 0.3, s.x.shape) * dx
+    with W.Simulation.from_state(p, state_dict(s)) as sim:
+        sim.force()
+        cnt, lists = sim.neighbours(cap=128)
+        st = sim.stats()
+    rc, rl = O.neighbours_brute(s.x, s.h.astype(np.float32), cap=128)
+    assert np.array_equal(cnt, rc)
+    assert np.array_equal(lists, np.where(np.arange(128)[None, :] < rc[:, None], rl, -1))
+    assert st["pairs_last"] == int(rc.sum()) and st["overflow"] == 0 and st["escaped"] == 0
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp64", 1e-12)])
+@pytest.mark.parametrize("dim,dx,geom", [(2, 0.02, {}), (3, 0.04, SMALL3)])
+def test_force_pass_matches_oracle(O, W, dim, dx, geom, precision, tol):
+    s, p, op = developed(O, W, dim, dx, 40, **geom)
+    pairs = s.force(op, 8)  # hydro only, on the same state
+    with W.Simulation.from_state(p, state_dict(s), precision=precision) as sim:
+        sim.force()
+        out = sim.download()
+        st = sim.stats()
+    assert rel(out["acc"], s.a) <= tol
+    assert rel(out["div_v"], s.divv) <= tol
+    assert np.allclose(out["dt"], s.dt_i, rtol=1e-6 if precision == "fp32" else 1e-13)
+    assert st["pairs_last"] == pairs
+
+
+def test_force_pass_with_variable_h_and_m(O, W):
+    s, p, op = developed(O, W, 3, 0.05, 10, **SMALL3)
+    rng = np.random.default_rng(5)
+    s.h *= 1 + 0.15 * rng.uniform(-1, 1, s.n)
+    s.mass *= 1 + 0.3 * rng.uniform(-1, 1, s.n)
+    s.force(op, 8)
+    with W.Simulation.from_state(p, state_dict(s)) as sim:
+        sim.force()
+        out = sim.download()
+    assert rel(out["acc"], s.a) <= 1e-5
+    assert rel(out["div_v"], s.divv) <= 1e-5
+
+
+def test_steps_match_oracle_fp64(O, W):
+    # 2D dam break, 30 full steps in FP64 on both sides: trajectories agree
+    # to rounding (only summation order differs)
+    p = W.dam_break_params(2)
+    s = O.State.dam_break(2, 0.02, c0=p.c0, g=p.g)
+    op = O.sim_params(2, alpha=p.alpha, c_cfl=p.c_cfl, c0=p.c0, g=p.g, nu=p.nu)
+    st0 = state_dict(s)
+    st0 = {k: v.copy() for k, v in st0.items()}
+    s.init(op, 8)
+    with W.Simulation.from_state(p, st0, precision="fp64") as sim:
+        sim.init()
+        sim.step(30)
+        out = sim.download()
+        stats = sim.stats()
+    for _ in range(30):
+        s.step(op, 8)
+    assert stats["step"] == 30
+    assert stats["time"] == pytest.approx(s.time, rel=1e-12)
+    disp_ref = s.x - st0["pos"]
+    assert rel(out["pos"] - st0["pos"], disp_ref) < 1e-9
+    assert rel(out["vel"], s.v) < 1e-9
+    assert np.allclose(out["dens"], s.dens, rtol=1e-12)
+
+
+def test_steps_match_oracle_fp32(O, W):
+    # same run with the FP32 force: the FP64 integration keeps the
+    # trajectory within 1e-4 (relative displacement) of the oracle's
+    p = W.dam_break_params(2)
+    s = O.State.dam_break(2, 0.02, c0=p.c0, g=p.g)
+    op = O.sim_params(2, alpha=p.alpha, c_cfl=p.c_cfl, c0=p.c0, g=p.g, nu=p.nu)
+    st0 = {k: v.copy() for k, v in state_dict(s).items()}
+    s.init(op, 8)
+    with W.Simulation.from_state(p, st0, precision="fp32") as sim:
+        sim.init()
+        sim.step(30)
+        out = sim.download()
+    for _ in range(30):
+        s.step(op, 8)
+    assert rel(out["pos"] - st0["pos"], s.x - st0["pos"]) < 1e-4
+    assert rel(out["vel"], s.v) < 1e-4
+    assert rel(out["dens"] - st0["dens"], s.dens - st0["dens"]) < 1e-4
+
+
+def test_free_fall_closed_form(W):
+    p = W.dam_break_params(3)
+    st = dict(pos=np.array([[0.3], [0.2], [5.0]]), vel=np.zeros((3, 1)), dens=np.array([1000.0]),
+              mass=np.array([1e-3]), h=np.array([0.01]), type=np.array([0], np.int32))
+    with W.Simulation.from_state(p, st) as sim:
+        sim.init()
+        sim.step(50)
+        out = sim.download()
+        t = sim.stats()["time"]
+    dt = min(p.c_cfl * 0.01 / p.c0, 0.25 * np.sqrt(0.01 / p.g))
+    assert t == pytest.approx(50 * dt, rel=1e-12)
+    assert out["pos"][2, 0] == pytest.approx(5.0 - 0.5 * p.g * t * t, rel=1e-12)
+    assert out["vel"][2, 0] == pytest.approx(-p.g * t, rel=1e-12)
+
+
+def test_deterministic_and_walls_fixed(O, W):
+    p = W.dam_break_params(2)
+    st = W.dam_break_state(W.dam_break_desc(2, 0.02), p)
+    outs = []
+    for _ in range(2):
+        with W.Simulation.from_state(p, st) as sim:
+            sim.init()
+            sim.step(25)
+            outs.append(sim.download())
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    w = st["type"] == 1
+    assert np.array_equal(outs[0]["pos"][:, w], st["pos"][:, w])
+
+
+def test_diagnostics_match_oracle(O, W):
+    s, p, op = developed(O, W, 2, 0.02, 20)
+    with W.Simulation.from_state(p, state_dict(s)) as sim:
+        d = sim.diagnostics(zfront=0.04, xcol=0.04)
+    ref = s.conservation(op)
+    got = [d[k] for k in ("mass", "mom_x", "mom_y", "kinetic", "potential", "internal")]
+    exp = [ref[0], ref[1], ref[2], ref[4], ref[5], ref[6]]
+    assert np.allclose(got, exp, rtol=1e-12, atol=1e-12 * np.abs(exp).max())
+    assert d["front"] == s.dam_front(0.04)
+    fl = (s.type == 0) & (s.x[0] < 0.04)
+    assert d["column"] == s.x[1][fl].max()

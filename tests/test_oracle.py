@@ -1,0 +1,406 @@
+"""Pins the FP64 oracle (oracle/wcsph_oracle.c) against closed forms,
+invariants and brute force. The reference's own build needs FDPS, which is
+absent, so these checks are what make the oracle trustworthy; each is chosen
+so a plausible mistake (wrong constant, dropped term, sign or index error)
+fails it. None of them retypes the oracle's formula.
+"""
+import numpy as np
+import pytest
+from scipy import integrate
+
+RNG = np.random.default_rng(20261203)
+
+
+# --------------------------------------------------------------------------
+# cubic spline, src/kernel.h:27-55
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("h", [0.013, 1.0, 2.7])
+def test_kernel_normalisation(O, dim, h):
+    # int W dV = 1 pins C_D and the support H = 2h
+    f = (lambda r: 2 * np.pi * r * O.kernel_W(dim, r, h)) if dim == 2 else \
+        (lambda r: 4 * np.pi * r * r * O.kernel_W(dim, r, h))
+    val, _ = integrate.quad(f, 0, 2 * h, points=[h], epsabs=1e-13, epsrel=1e-12)
+    assert abs(val - 1.0) < 1e-10
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_kernel_support_and_smoothness(O, dim):
+    h = 0.7
+    assert O.kernel_W(dim, 2 * h, h) == 0.0
+    assert O.kernel_W(dim, 2.3 * h, h) == 0.0
+    assert O.kernel_gradW_factor(dim, 2.01 * h, h) == 0.0
+    # C1 at s = 1/2 (r = h): one-sided derivatives agree
+    e = 1e-6 * h
+    dl = (O.kernel_W(dim, h, h) - O.kernel_W(dim, h - e, h)) / e
+    dr = (O.kernel_W(dim, h + e, h) - O.kernel_W(dim, h, h)) / e
+    assert abs(dl - dr) < 1e-4 * abs(dl)
+    # W decreasing on (0, 2h)
+    rs = np.linspace(0.01, 1.99, 200) * h
+    w = [O.kernel_W(dim, r, h) for r in rs]
+    assert np.all(np.diff(w) < 0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gradW_is_derivative_of_W(O, dim):
+    # gradW = F dr with F r = dW/dr: pins the gradient against the normalised W
+    h = 0.9
+    for r in np.linspace(0.05, 1.98, 40) * h:
+        e = 1e-6 * h
+        dWdr = (O.kernel_W(dim, r + e, h) - O.kernel_W(dim, r - e, h)) / (2 * e)
+        F = O.kernel_gradW_factor(dim, r, h)
+        assert abs(F * r - dWdr) < 1e-6 * max(1.0, abs(dWdr)) * O.kernel_W(dim, 0.0, h)
+
+
+# --------------------------------------------------------------------------
+# Tait / Murnaghan EoS, src/EoS.h:75-90
+# --------------------------------------------------------------------------
+def test_eos_special_values(O):
+    rho0, c0, g = 1000.0, 62.6, 7.0
+    B = c0 * c0 * rho0 / g
+    assert O.eos_pressure(rho0, rho0, c0, g) == 0.0
+    assert abs(O.eos_soundspeed(rho0, rho0, c0, g) - c0) < 1e-12 * c0
+    assert O.eos_pressure(0.98 * rho0, rho0, c0, g) == 0.0  # clamped (tension removed)
+    assert abs(O.eos_pressure(1.01 * rho0, rho0, c0, g) - B * (1.01 ** 7 - 1)) < 1e-9 * B
+    # c^2 = dP/drho of the unclamped Tait law, checked at rho > rho0
+    for rho in (1000.5, 1003.0, 1020.0):
+        e = 1e-4
+        dP = (O.eos_pressure(rho + e, rho0, c0, g) - O.eos_pressure(rho - e, rho0, c0, g)) / (2 * e)
+        assert abs(O.eos_soundspeed(rho, rho0, c0, g) ** 2 - dP) < 1e-6 * dP
+    assert O.eos_soundspeed(1e-30, rho0, c0, g) == pytest.approx(1e-8)  # floor sqrt(1e-16)
+
+
+# --------------------------------------------------------------------------
+# the pair functor, src/force.h:31-60
+# --------------------------------------------------------------------------
+def make_cloud(O, dim, n, *, spread=1.0, hvar=0.0, mvar=0.0, vscale=1.0, P=True, nu=1e-6, seed=0):
+    rng = np.random.default_rng(seed)
+    dx = spread / n ** (1.0 / dim)
+    pos = rng.uniform(0, spread, (n, dim))
+    vel = rng.normal(0, vscale, (n, dim))
+    h = 1.3 * dx * (1 + hvar * rng.uniform(-1, 1, n))
+    dens = 1000.0 * (1 + 0.01 * rng.normal(size=n))
+    mass = dens * dx ** dim * (1 + mvar * rng.uniform(-1, 1, n))
+    c0 = 30.0
+    pres = np.array([O.eos_pressure(d, 1000.0, c0, 7.0) for d in dens]) if P else np.zeros(n)
+    snds = np.array([O.eos_soundspeed(d, 1000.0, c0, 7.0) for d in dens])
+    epi = np.zeros(n, O.epi_dtype(dim))
+    epj = np.zeros(n, O.epj_dtype(dim))
+    for a in (epi, epj):
+        a["pos"], a["vel"], a["smth"], a["dens"], a["pres"], a["snds"], a["visc"] = pos, vel, h, dens, pres, snds, nu
+        a["tag"] = np.arange(n)
+    epj["mass"] = mass
+    return epi, epj
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_two_particles_pressure(O, dim):
+    # two particles at rest: acc_i = -m_j (P_i/rho_i^2 + P_j/rho_j^2) dW/dr(hbar) e_ij
+    epi, epj = make_cloud(O, dim, 2, seed=1, P=True, nu=0.0)
+    epi["vel"] = 0
+    epj["vel"] = 0
+    epi["pos"][0] = 0
+    epi["pos"][1] = 0
+    epi["pos"][1][0] = 0.7
+    epj["pos"] = epi["pos"]
+    for a in (epi, epj):
+        a["smth"] = [0.5, 0.6]
+        a["dens"] = [1010.0, 1004.0]
+        a["pres"] = [3.0e4, 1.5e4]
+    res = O.hydro_force(O.force_params(dim), epi, epj)
+    hbar, r, e = 0.55, 0.7, 1e-7
+    dWdr = (O.kernel_W(dim, r + e, hbar) - O.kernel_W(dim, r - e, hbar)) / (2 * e)
+    coef = 3.0e4 / 1010.0 ** 2 + 1.5e4 / 1004.0 ** 2
+    # i=0 sits at x=0, j=1 at x=0.7: dr = -0.7 e_x, so acc_0 = +m_1 coef dWdr (dWdr < 0): pushed to -x
+    expect0 = -epj["mass"][1] * coef * dWdr * (-1.0)
+    assert res["acc"][0][0] == pytest.approx(expect0, rel=1e-6)
+    assert res["acc"][0][0] < 0  # repulsion
+    assert np.all(res["acc"][0][1:] == 0)
+    assert res["div_v"][0] == 0.0
+    # dt quirk: v_sig stays 0, so dt = C_CFL h / c (src/force.h:47,56)
+    assert res["dt"][0] == pytest.approx(0.3 * 0.5 / epi["snds"][0], rel=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_two_particles_artificial_viscosity(O, dim):
+    # approaching pair with P = 0, nu = 0: acc_i = -m_j Pi dW/dr e_ij with the
+    # Monaghan Pi = -alpha cbar mu / rhobar, mu = hbar v.r / (r^2 + eps h_i h_j)
+    epi, epj = make_cloud(O, dim, 2, seed=2, P=False, nu=0.0)
+    for a in (epi, epj):
+        a["pos"] = 0
+        a["pos"][1][0] = 0.5
+        a["vel"] = 0
+        a["vel"][0][0] = 2.0   # i moves towards j
+        a["smth"] = [0.4, 0.4]
+        a["dens"] = [1000.0, 1000.0]
+        a["snds"] = [30.0, 30.0]
+    alpha, eps = 0.25, 0.01
+    res = O.hydro_force(O.force_params(dim, alpha=alpha), epi, epj)
+    r, h, e = 0.5, 0.4, 1e-7
+    vr = (2.0 - 0.0) * (0.0 - 0.5)  # (v_i - v_j).(x_i - x_j) < 0: approaching
+    mu = h * vr / (r * r + eps * h * h)
+    Pi = -alpha * 30.0 * mu / 1000.0
+    dWdr = (O.kernel_W(dim, r + e, h) - O.kernel_W(dim, r - e, h)) / (2 * e)
+    expect = -epj["mass"][1] * Pi * dWdr * (-1.0)
+    assert res["acc"][0][0] == pytest.approx(expect, rel=1e-6)
+    assert res["acc"][0][0] < 0  # decelerates the approaching particle
+    # receding pair: no artificial viscosity
+    for a in (epi, epj):
+        a["vel"][0][0] = -2.0
+    res = O.hydro_force(O.force_params(dim, alpha=alpha), epi, epj)
+    assert res["acc"][0][0] == 0.0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_momentum_antisymmetry(O, dim):
+    # every pair term is antisymmetric in (i,j) (symmetric h, m-weighting),
+    # so sum_i m_i a_i = 0 when each i sees all particles as j
+    epi, epj = make_cloud(O, dim, 300, hvar=0.2, mvar=0.3, seed=3)
+    res = O.hydro_force(O.force_params(dim, alpha=0.3, beta=0.5), epi, epj)
+    ma = (epj["mass"][:, None] * res["acc"]).sum(0)
+    scale = (epj["mass"][:, None] * np.abs(res["acc"])).sum(0)
+    assert np.all(np.abs(ma) < 1e-12 * scale.max())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_dissipation_sign(O, dim):
+    # with P = 0 only artificial and laminar viscosity act: they must remove
+    # kinetic energy, sum_i m_i v_i.a_i <= 0
+    epi, epj = make_cloud(O, dim, 300, P=False, nu=1e-3, seed=4, hvar=0.1)
+    res = O.hydro_force(O.force_params(dim, alpha=0.5, beta=1.0), epi, epj)
+    power = (epj["mass"][:, None] * epi["vel"] * res["acc"]).sum()
+    assert power < 0
+    # the laminar term alone (alpha = 0) is dissipative too
+    res = O.hydro_force(O.force_params(dim, alpha=0.0), epi, epj)
+    assert (epj["mass"][:, None] * epi["vel"] * res["acc"]).sum() < 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_galilean_and_order_invariance(O, dim):
+    epi, epj = make_cloud(O, dim, 200, seed=5, hvar=0.2)
+    p = O.force_params(dim, alpha=0.2, beta=0.3, vsig_mode=1)
+    r0 = O.hydro_force(p, epi, epj)
+    # uniform velocity shift
+    e2, j2 = epi.copy(), epj.copy()
+    e2["vel"] += 3.7
+    j2["vel"] += 3.7
+    r1 = O.hydro_force(p, e2, j2)
+    for k in ("acc", "div_v", "dt"):
+        assert np.allclose(r1[k], r0[k], rtol=1e-9, atol=1e-9 * np.abs(r0[k]).max())
+    # permuted j list plus far-away extra particles (FDPS superset lists)
+    perm = RNG.permutation(len(epj))
+    far = epj[:20].copy()
+    far["pos"] += 100.0
+    j3 = np.concatenate([epj[perm], far])
+    r2 = O.hydro_force(p, epi, j3)
+    for k in ("acc", "div_v"):
+        assert np.allclose(r2[k], r0[k], rtol=1e-11, atol=1e-11 * np.abs(r0[k]).max())
+
+
+def test_rotation_invariance_3d(O):
+    epi, epj = make_cloud(O, 3, 200, seed=6, hvar=0.2)
+    p = O.force_params(3, alpha=0.2)
+    r0 = O.hydro_force(p, epi, epj)
+    q, _ = np.linalg.qr(RNG.normal(size=(3, 3)))
+    e2, j2 = epi.copy(), epj.copy()
+    for a in (e2, j2):
+        a["pos"] = a["pos"] @ q.T
+        a["vel"] = a["vel"] @ q.T
+    r1 = O.hydro_force(p, e2, j2)
+    assert np.allclose(r1["acc"], r0["acc"] @ q.T, atol=1e-9 * np.abs(r0["acc"]).max())
+    assert np.allclose(r1["div_v"], r0["div_v"], atol=1e-9 * np.abs(r0["div_v"]).max())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_lattice_interior_gradients(O, dim):
+    # on a lattice interior, the functor's div_v of v = A x tends to trace(A)
+    # and its pressure term of a linear P tends to -grad P / rho (SPH
+    # consistency at h = 1.3 dx, a few percent): pins signs and scales.
+    dx = 0.02
+    m = 13 if dim == 3 else 31
+    g = np.stack(np.meshgrid(*[np.arange(m)] * dim, indexing="ij"), -1).reshape(-1, dim) * dx
+    n = len(g)
+    c = np.argmin(((g - g.mean(0)) ** 2).sum(1))
+    A = RNG.normal(size=(dim, dim))
+    epj = np.zeros(n, O.epj_dtype(dim))
+    epj["pos"] = g
+    epj["vel"] = g @ A.T
+    epj["smth"], epj["dens"], epj["snds"], epj["visc"] = 1.3 * dx, 1000.0, 30.0, 0.0
+    epj["mass"] = 1000.0 * dx ** dim
+    gradP = np.array([3.0e4, -2.0e4, 1.0e4][:dim])
+    epj["pres"] = 5.0e4 + g @ gradP
+    epi = np.zeros(1, O.epi_dtype(dim))
+    for k in ("pos", "vel", "smth", "dens", "pres", "snds", "visc"):
+        epi[k] = epj[k][c]
+    res = O.hydro_force(O.force_params(dim, alpha=0.0), epi, epj)
+    assert res["div_v"][0] == pytest.approx(np.trace(A), rel=0.05, abs=0.05 * np.abs(A).max())
+    # pressure part only: rerun with v = 0
+    epj["vel"] = 0
+    epi["vel"] = 0
+    res = O.hydro_force(O.force_params(dim, alpha=0.0), epi, epj)
+    assert np.allclose(res["acc"][0], -gradP / 1000.0, rtol=0.05, atol=0.05 * np.abs(gradP).max() / 1000)
+
+
+# --------------------------------------------------------------------------
+# neighbour sets, FDPS-style walks
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dim", [2, 3])
+def test_neighbour_brute_vs_distance(O, dim):
+    n = 800
+    pos = RNG.uniform(0, 1, (dim, n))
+    h = (0.05 * (1 + 0.3 * RNG.uniform(-1, 1, n))).astype(np.float32)
+    counts, lists = O.neighbours_brute(pos, h, cap=256)
+    d = np.sqrt(((pos[:, :, None] - pos[:, None, :]) ** 2).sum(0))
+    H = h[:, None].astype(np.float64) + h[None, :]
+    exact = (d < H) & (d > 0)
+    # the FP32 predicate may only differ from the FP64 distance within rounding of the boundary
+    border = np.abs(d - H) < 1e-5 * H
+    got = np.zeros_like(exact)
+    for i in range(n):
+        got[i, lists[i, :counts[i]]] = True
+    assert np.all((got == exact) | border)
+    assert np.array_equal(got, got.T)  # the predicate is symmetric
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_walks_are_supersets(O, dim):
+    n = 3000
+    pos = RNG.uniform(0, 1, (dim, n))
+    h = np.full(n, 0.02, np.float32)
+    go, gi, no, ni = O.make_walks(pos, r_search=2 * 0.02, group_max=128)
+    assert np.array_equal(np.sort(gi), np.arange(n))  # the groups partition the particles
+    assert np.all(np.diff(go) <= 128)
+    counts, lists = O.neighbours_brute(pos, h, cap=128)
+    for w in range(len(go) - 1):
+        js = set(ni[no[w]:no[w + 1]].tolist())
+        for i in gi[go[w]:go[w + 1]]:
+            assert set(lists[i, :counts[i]].tolist()) <= js
+            assert i in js
+
+
+# --------------------------------------------------------------------------
+# initial condition, integrator, time step
+# --------------------------------------------------------------------------
+def test_dam_break_counts(O):
+    s2 = O.State.dam_break(2, 0.02)
+    assert (s2.type == 0).sum() == 5000 and (s2.type == 1).sum() == 1518
+    s3 = O.State.dam_break(3, 0.0126)
+    assert (s3.type == 0).sum() == 502440 and (s3.type == 1).sum() == 562938
+
+
+def test_free_fall_is_exact(O):
+    # one isolated fluid particle: no hydro force, dt = min(C h / c, 0.25 sqrt(h/g)),
+    # and KDK integrates constant gravity exactly: z = z0 - g t^2 / 2, v = -g t
+    s = O.State(3, 1)
+    s.x[:, 0] = [0.3, 0.2, 5.0]
+    s.dens[0], s.mass[0], s.h[0] = 1000.0, 1e-3, 0.01
+    p = O.sim_params(3)
+    s.init(p)
+    dt_expect = min(0.25 * 0.01 / p.c0, 0.25 * np.sqrt(0.01 / p.g))
+    assert s.global_dt(p) == pytest.approx(dt_expect, rel=1e-14)
+    for _ in range(50):
+        s.step(p)
+    t = s.time
+    assert t == pytest.approx(50 * dt_expect, rel=1e-12)
+    assert s.x[2, 0] == pytest.approx(5.0 - 0.5 * p.g * t * t, rel=1e-12)
+    assert s.v[2, 0] == pytest.approx(-p.g * t, rel=1e-12)
+    assert s.x[0, 0] == 0.3 and s.v[0, 0] == 0.0
+
+
+def test_walls_do_not_move(O):
+    s = O.State.dam_break(2, 0.05)
+    p = O.sim_params(2)
+    s.init(p)
+    x0 = s.x.copy()
+    for _ in range(20):
+        s.step(p)
+    w = s.type == 1
+    assert np.array_equal(s.x[:, w], x0[:, w])
+    assert np.all(s.v[:, w] == 0)
+    assert np.any(s.x[:, ~w] != x0[:, ~w])
+
+
+def test_hydrostatic_start_is_near_balance(O):
+    # the hydrostatic initial density makes the pressure force hold the column
+    # against gravity: the interior of the 2D column barely accelerates
+    s = O.State.dam_break(2, 0.02)
+    p = O.sim_params(2)
+    s.init(p)
+    x, y = s.x
+    inner = (s.type == 0) & (x > 0.2) & (x < 0.8) & (y > 0.2) & (y < 1.6)
+    a_net = s.a[:, inner]  # includes gravity
+    assert np.abs(a_net[1]).mean() < 0.1 * p.g
+
+
+def test_cell_force_equals_all_pairs(O):
+    # the whole-step oracle's cell-list force equals the plain functor over all particles
+    s = O.State.dam_break(2, 0.05)
+    p = O.sim_params(2)
+    s.init(p)
+    for _ in range(5):
+        s.step(p)
+    n = s.n
+    epi = np.zeros(n, O.epi_dtype(2))
+    epj = np.zeros(n, O.epj_dtype(2))
+    for a in (epi, epj):
+        a["pos"], a["vel"], a["smth"], a["dens"], a["pres"], a["snds"], a["visc"] = \
+            s.x.T, s.v.T, s.h, s.dens, s.pres, s.snds, p.nu
+    epj["mass"] = s.mass
+    fp = O.force_params(2, alpha=p.alpha, c_cfl=p.c_cfl)
+    ref = O.hydro_force(fp, epi, epj)
+    s.force(p)  # hydro only
+    assert np.allclose(s.a.T, ref["acc"], rtol=1e-10, atol=1e-10 * np.abs(ref["acc"]).max())
+    assert np.allclose(s.divv, ref["div_v"], rtol=1e-10, atol=1e-10 * np.abs(ref["div_v"]).max())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_signal_velocity_modes(O, dim):
+    # approaching pair: mu < 0. Mode 0 (src/force.h:47) keeps v_sig = max(0, mu) = 0;
+    # mode 1 (src/kernel.cu:263) uses -mu, so dt = C h / (c + |mu|)
+    epi, epj = make_cloud(O, dim, 2, seed=7, P=False, nu=0.0)
+    for a in (epi, epj):
+        a["pos"] = 0
+        a["pos"][1][0] = 0.5
+        a["vel"] = 0
+        a["vel"][0][0] = 2.0
+        a["smth"], a["dens"], a["snds"] = 0.4, 1000.0, 30.0
+    mu = 0.4 * (2.0 * -0.5) / (0.25 + 0.01 * 0.16)
+    r0 = O.hydro_force(O.force_params(dim, vsig_mode=0), epi, epj)
+    r1 = O.hydro_force(O.force_params(dim, vsig_mode=1), epi, epj)
+    assert r0["dt"][0] == pytest.approx(0.3 * 0.4 / 30.0, rel=1e-14)
+    assert r1["dt"][0] == pytest.approx(0.3 * 0.4 / (30.0 - mu), rel=1e-12)
+
+
+def test_step_composition(O):
+    # one step of src/main.cpp:89-133 composed by hand from the (separately
+    # pinned) functor: kick-drift with the old acceleration, force on the
+    # predicted velocity v + dt a, gravity, final kick v = v_half + dt/2 a
+    s = O.State.dam_break(2, 0.1, col_w=0.5, col_h=0.6, tank_l=1.0, tank_h=1.0)
+    p = O.sim_params(2, alpha=0.3)
+    s.init(p)
+    for _ in range(3):
+        s.step(p)
+    x0, v0, a0, rho0, dv0 = s.x.copy(), s.v.copy(), s.a.copy(), s.dens.copy(), s.divv.copy()
+    dt = s.global_dt(p)
+    s.step(p)
+    fl = s.type == 0
+    vh = np.where(fl, v0 + 0.5 * dt * a0, 0.0)
+    x1 = np.where(fl, x0 + dt * vh, x0)
+    vp = np.where(fl, v0 + dt * a0, v0)
+    rho1 = rho0 - rho0 * dt * dv0
+    n = s.n
+    epi = np.zeros(n, O.epi_dtype(2))
+    epj = np.zeros(n, O.epj_dtype(2))
+    P = np.array([O.eos_pressure(r, p.rho0, p.c0, p.gamma) for r in rho1])
+    c = np.array([O.eos_soundspeed(r, p.rho0, p.c0, p.gamma) for r in rho1])
+    for a in (epi, epj):
+        a["pos"], a["vel"], a["smth"], a["dens"], a["pres"], a["snds"], a["visc"] = x1.T, vp.T, s.h, rho1, P, c, p.nu
+    epj["mass"] = s.mass
+    ref = O.hydro_force(O.force_params(2, alpha=p.alpha, c_cfl=p.c_cfl), epi, epj)
+    acc = ref["acc"].T.copy()
+    acc[1] -= p.g
+    v1 = np.where(fl, vh + 0.5 * dt * acc, vp)
+    assert np.allclose(s.x, x1, rtol=0, atol=1e-14)
+    assert np.allclose(s.a, acc, rtol=1e-10, atol=1e-10 * np.abs(acc).max())
+    assert np.allclose(s.v, v1, rtol=1e-10, atol=1e-12)
+    assert np.allclose(s.dens, rho1, rtol=1e-14)

@@ -1,0 +1,221 @@
+"""MI355X (gfx950) hydro force path for weakly compressible SPH.
+
+Python mirror of the C-ABI in include/wcsph_abi.h (libwcsph.so):
+
+* ``HydrDispatchKernel`` / ``HydrRetrieveKernel`` keep the reference's FDPS
+  multiwalk callback names and argument meaning (src/prototype.h:7-10,
+  src/kernel.cu:289,383) and, like them, return an int status;
+* ``Simulation`` drives the device-resident step (cell list, neighbour and
+  force passes, FP64 kick-drift-kick), replacing the loop of
+  src/main.cpp:89-133.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._abi import (PREC_FP32, PREC_FP64, PREC_MIXED, PRECISIONS, DamBreakDesc, Params, SimConfig, SimStats,
+                   WcsphError, check, epi_dtype, epj_dtype, lib, ptr, res_dtype)
+
+__all__ = ["Params", "SimConfig", "WcsphError", "reference_params", "dam_break_params", "init", "finalize",
+           "HydrDispatchKernel", "HydrRetrieveKernel", "last_kernel_ms", "epi_dtype", "epj_dtype", "res_dtype",
+           "dam_break_desc", "dam_break_state", "Simulation", "PREC_FP32", "PREC_MIXED", "PREC_FP64"]
+
+
+def reference_params(dim, device=0):
+    """Constants of the reference CPU functor and its dam-break problem."""
+    p = Params()
+    check(lib().wcsph_params_reference(dim, C.byref(p)))
+    p.device = device
+    return p
+
+
+def dam_break_params(dim, *, col_h=2.0, alpha=0.1, beta=0.0, c_cfl=0.25, g=9.81, rho0=1000.0, gamma=7.0,
+                     nu=1.0e-6, c0=None, rho_floor=0.0, vsig_mode=0, eps=0.01, device=0):
+    """Physics of the dam-break configurations: c0 = 10 sqrt(2 g H), alpha = 0.1, CFL 0.25, g = 9.81."""
+    p = reference_params(dim, device)
+    p.alpha, p.beta, p.c_cfl, p.g, p.rho0, p.gamma, p.nu = alpha, beta, c_cfl, g, rho0, gamma, nu
+    p.c0 = 10.0 * np.sqrt(2.0 * g * col_h) if c0 is None else c0
+    p.rho_floor, p.vsig_mode, p.eps = rho_floor, vsig_mode, eps
+    return p
+
+
+# ----------------------------------------------------------------------------
+# FDPS multiwalk drop-in
+# ----------------------------------------------------------------------------
+def init(params):
+    check(lib().wcsph_init(C.byref(params)))
+    _dim_hint[0] = params.dim
+
+
+def finalize():
+    check(lib().wcsph_finalize())
+
+
+def _ptr_array(arrays):
+    return (C.c_void_p * max(len(arrays), 1))(*[a.ctypes.data if len(a) else None for a in arrays])
+
+
+def HydrDispatchKernel(tag, n_walk, epi, n_epi, epj, n_epj):
+    """GPU_WC::HydrDispatchKernel (src/kernel.cu:289). epi/epj: sequences of
+    structured arrays in the EPI/EPJ layouts; returns 0 or an error code."""
+    dim = epi[0].dtype["pos"].shape[0] if n_walk else lib_dim()
+    fn = lib().wcsph_hydro_dispatch_3d if dim == 3 else lib().wcsph_hydro_dispatch_2d
+    ni = np.ascontiguousarray(n_epi, np.int32)
+    nj = np.ascontiguousarray(n_epj, np.int32)
+    _keep = (_ptr_array(epi), _ptr_array(epj))
+    return fn(tag, n_walk, C.cast(_keep[0], C.c_void_p), ptr(ni), C.cast(_keep[1], C.c_void_p), ptr(nj))
+
+
+def HydrRetrieveKernel(tag, n_walk, ni, force):
+    """GPU_WC::HydrRetrieveKernel (src/kernel.cu:383). force: sequence of
+    writable RESULT arrays; returns 0 or an error code."""
+    dim = force[0].dtype["acc"].shape[0] if n_walk else lib_dim()
+    fn = lib().wcsph_hydro_retrieve_3d if dim == 3 else lib().wcsph_hydro_retrieve_2d
+    n = np.ascontiguousarray(ni, np.int32)
+    arr = _ptr_array(force)
+    return fn(tag, n_walk, ptr(n), C.cast(arr, C.c_void_p))
+
+
+_dim_hint = [3]
+
+
+def lib_dim():
+    return _dim_hint[0]
+
+
+def last_kernel_ms(tag=0):
+    ms = C.c_float()
+    check(lib().wcsph_hydro_last_kernel_ms(tag, C.byref(ms)))
+    return ms.value
+
+
+def last_error():
+    return lib().wcsph_last_error().decode()
+
+
+# ----------------------------------------------------------------------------
+# dam-break initial condition (generated by the library, on the host)
+# ----------------------------------------------------------------------------
+def dam_break_desc(dim, dx, *, col_w=1.0, col_h=2.0, depth=0.5, tank_l=4.0, tank_h=3.0, nwall=3, hfac=1.3):
+    return DamBreakDesc(dim, nwall, dx, col_w, col_h, depth, tank_l, tank_h, hfac)
+
+
+def dam_break_state(desc, params):
+    """Host SoA arrays of the dam-break lattice: dict(pos, vel, dens, mass, h, type)."""
+    nf, nw = C.c_int64(), C.c_int64()
+    check(lib().wcsph_dam_break_count(C.byref(desc), C.byref(nf), C.byref(nw)))
+    n = nf.value + nw.value
+    D = desc.dim
+    st = dict(pos=np.zeros((D, n)), vel=np.zeros((D, n)), dens=np.zeros(n), mass=np.zeros(n), h=np.zeros(n),
+              type=np.zeros(n, np.int32))
+    check(lib().wcsph_dam_break_fill(C.byref(desc), C.byref(params), *[ptr(st[k]) for k in
+                                                                       ("pos", "vel", "dens", "mass", "h", "type")]))
+    st["n_fluid"], st["n_wall"] = nf.value, nw.value
+    return st
+
+
+# ----------------------------------------------------------------------------
+# device-resident simulation
+# ----------------------------------------------------------------------------
+class Simulation:
+    """One device-resident SPH system (all particle data in HBM)."""
+
+    def __init__(self, params, n, *, precision="fp32", max_neighbours=128, box_lo=None, box_hi=None):
+        self.params = params
+        self.dim = params.dim
+        self.n = int(n)
+        cfg = SimConfig()
+        cfg.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        cfg.max_neighbours = max_neighbours
+        if box_lo is not None:
+            for k in range(self.dim):
+                cfg.box_lo[k], cfg.box_hi[k] = box_lo[k], box_hi[k]
+        self.cfg = cfg
+        h = C.c_void_p()
+        check(lib().wcsph_sim_create(C.byref(params), C.byref(cfg), self.n, C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def from_state(cls, params, state, **kw):
+        sim = cls(params, len(state["dens"]), **kw)
+        sim.upload(state)
+        return sim
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib().wcsph_sim_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, st):
+        D, n = self.dim, self.n
+        arrs = [np.ascontiguousarray(st["pos"], np.float64).reshape(D, n),
+                np.ascontiguousarray(st["vel"], np.float64).reshape(D, n),
+                np.ascontiguousarray(st["dens"], np.float64), np.ascontiguousarray(st["mass"], np.float64),
+                np.ascontiguousarray(st["h"], np.float64), np.ascontiguousarray(st["type"], np.int32)]
+        check(lib().wcsph_sim_upload(self._h, *[ptr(a) for a in arrs]))
+
+    def init(self):
+        check(lib().wcsph_sim_init(self._h))
+
+    def step(self, nsteps=1):
+        check(lib().wcsph_sim_step(self._h, nsteps))
+
+    def force(self):
+        check(lib().wcsph_sim_force(self._h))
+
+    def sync(self):
+        check(lib().wcsph_sim_sync(self._h))
+
+    def stats(self):
+        s = SimStats()
+        check(lib().wcsph_sim_get_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in SimStats._fields_}
+
+    def download(self):
+        D, n = self.dim, self.n
+        out = dict(pos=np.zeros((D, n)), vel=np.zeros((D, n)), dens=np.zeros(n), acc=np.zeros((D, n)),
+                   div_v=np.zeros(n), dt=np.zeros(n))
+        check(lib().wcsph_sim_download(self._h, *[ptr(out[k]) for k in ("pos", "vel", "dens", "acc", "div_v", "dt")]))
+        return out
+
+    def neighbours(self, cap=0):
+        counts = np.zeros(self.n, np.int32)
+        lists = np.zeros((self.n, cap), np.int32) if cap else None
+        check(lib().wcsph_sim_neighbours(self._h, ptr(counts), ptr(lists), cap))
+        return (counts, lists) if cap else counts
+
+    def diagnostics(self, zfront, xcol):
+        out = np.zeros(9)
+        check(lib().wcsph_sim_diagnostics(self._h, zfront, xcol, ptr(out)))
+        keys = ["mass", "mom_x", "mom_y", "mom_z", "kinetic", "potential", "internal", "front", "column"]
+        return dict(zip(keys, out))
+
+    def set_timing(self, enable=True):
+        check(lib().wcsph_sim_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self):
+        """Summed device ms since set_timing(True): stages and whole steps."""
+        out = np.zeros(5)
+        n = C.c_int64()
+        check(lib().wcsph_sim_timing(self._h, ptr(out), C.byref(n)))
+        keys = ["kick_bin_reorder", "neighbour", "force", "final", "step"]
+        d = dict(zip(keys, out.tolist()))
+        d["steps"] = n.value
+        return d
+
+    def last_force_ms(self):
+        out = np.zeros(3, np.float32)
+        check(lib().wcsph_sim_last_force_ms(self._h, ptr(out)))
+        return {"binning": float(out[0]), "neighbour": float(out[1]), "force": float(out[2])}

@@ -1,0 +1,83 @@
+// Device pair interaction of the hydro force (pressure gradient, Monaghan
+// artificial viscosity, laminar viscosity, velocity divergence), written for
+// FP32 and FP64 from the reference functor's formula (src/force.h:38-57):
+//
+//   dr = r_i - r_j, dv = v_i - v_j, r2 = dr.dr            (skip if r2 <= 0)
+//   den = r2 + eps h_i h_j
+//   mu  = dr.dv < 0 ? (h_i + h_j)/2 dr.dv / den : 0
+//   Pi  = (-alpha cbar mu + beta mu^2) / rhobar            (beta = 0 at :46)
+//   gradW = F dr,  F = C_D g(s) / (H^(D+1) r),  H = h_i + h_j, s = r / H
+//           g(s) = -3 (1-s)+^2 + 12 (1/2-s)+^2             (src/kernel.h:49-55
+//           evaluated at the mean smoothing length, src/force.h:49)
+//   acc  -= m_j (P_i/rho_i^2 + P_j/rho_j^2 + Pi) gradW
+//   acc  += m_j 2 (nu_i + nu_j) (dr.gradW) / (rho_i + rho_j) / den dv
+//   divv -= m_j dv.gradW
+//
+// Outside the support F is exactly zero (both (.)+ terms vanish), so the
+// caller may skip the second half there; mu still feeds the signal velocity.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace wcsph {
+
+template <typename T>
+struct PairConsts {
+    T alpha, beta, eps, cnorm;  // cnorm = C_D: 80/(7 pi) in 2D, 16/pi in 3D (src/kernel.h:35-41)
+    int vsig_mode;
+};
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double fast_rsqrt(double x) { return 1.0 / sqrt(x); }
+
+template <typename T>
+struct IState {  // per-i constants of the inner loop
+    T h, rho, pr, c, nu;  // pr = P / rho^2
+};
+
+template <typename T>
+struct JState {
+    T h, rho, pr, c, nu, m;
+};
+
+// First half: regulariser, mu, and the signal-velocity update. Returns den.
+template <typename T>
+__device__ __forceinline__ T pair_mu(T r2, T drdv, const IState<T>& I, T hj, const PairConsts<T>& k,
+                                     T& mu, T& vsig) {
+    const T den = r2 + k.eps * (I.h * hj);
+    const T hs = I.h + hj;
+    mu = drdv < T(0) ? T(0.5) * hs * drdv * fast_rcp(den) : T(0);
+    const T vs = k.vsig_mode ? -mu : mu;
+    vsig = vs > vsig ? vs : vsig;
+    return den;
+}
+
+// Second half, valid for 0 < r2 < (h_i+h_j)^2.
+template <int D, typename T>
+__device__ __forceinline__ void pair_force(const T* dr, const T* dv, T r2, T drdv, T den, T mu,
+                                           const IState<T>& I, const JState<T>& J,
+                                           const PairConsts<T>& k, T* acc, T& divv) {
+    const T rho_s = I.rho + J.rho;
+    const T inv_rs = fast_rcp(rho_s);
+    const T cbar = T(0.5) * (I.c + J.c);
+    const T Pi = (-k.alpha * cbar * mu + k.beta * mu * mu) * (T(2) * inv_rs);  // 1/rhobar = 2/(rho_i+rho_j)
+    const T H = I.h + J.h;
+    const T invH = fast_rcp(H);
+    const T invr = fast_rsqrt(r2);
+    const T s = r2 * invr * invH;
+    const T a = s < T(1) ? T(1) - s : T(0);
+    const T b = s < T(0.5) ? T(0.5) - s : T(0);
+    const T g = T(-3) * a * a + T(12) * b * b;
+    const T invH2 = invH * invH;
+    const T invHD1 = D == 3 ? invH2 * invH2 : invH2 * invH;
+    const T mF = J.m * k.cnorm * g * invHD1 * invr;           // m_j F
+    const T pt = I.pr + J.pr + Pi;
+    const T vc = mF * T(2) * (I.nu + J.nu) * r2 * inv_rs * fast_rcp(den);
+    const T cp = -mF * pt;
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] += cp * dr[d] + vc * dv[d];
+    divv -= mF * drdv;
+}
+
+}  // namespace wcsph

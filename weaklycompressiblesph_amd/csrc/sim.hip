@@ -1,0 +1,576 @@
+// Device-resident simulation: host orchestration of the step kernels.
+//
+// One step (src/main.cpp:89-133 without the Shepard filter) is queued as:
+//   k_scalars(commit dt)      getTimeStepGlobal result of the previous step
+//   k_kick_drift              initialKick + fullDrift + predict (FP64)
+//   k_fill/k_bin/scan/scatter/k_cell_sort   cell list by counting sort
+//   k_reorder                 permute the state into cell order + EoS
+//   k_neigh                   neighbour lists (replaces the FDPS tree walk)
+//   k_force                   hydro force (replaces dispatch/retrieve)
+//   k_final                   gravity + finalKick + density floor + dt bound
+// Nothing returns to the host between steps; dt lives on the device.
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+#include "sim_kernels.hpp"
+
+using namespace wcsph;
+
+struct wcsph_sim {
+    wcsph_params p{};
+    wcsph_sim_config cfg{};
+    int D = 3;
+    int64_t n = 0;
+    int K = 128;
+    int device = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    Grid grid{};
+    int64_t ncell = 0, npart = 0;
+    Master m[2]{};
+    int cur = 0;
+    double *acc = nullptr, *divv = nullptr, *dti = nullptr;
+    float* hf = nullptr;
+    float4* q4 = nullptr;
+    double *pr64 = nullptr, *c64 = nullptr;  // FP64 path only
+    uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr,
+             *part = nullptr;
+    uint32_t *nbr = nullptr, *cnt = nullptr;
+    Scalars* sc = nullptr;
+    double* diag_part = nullptr;
+    double* diag_out = nullptr;
+    bool uploaded = false, forced = false;
+    bool timing = false;
+    std::vector<hipEvent_t> tev;  // 5 events per timed step, pooled
+    int64_t tsteps = 0;
+    std::vector<void*> allocs;
+
+    template <class T>
+    T* alloc(int64_t count) {
+        void* ptr = nullptr;
+        WCSPH_HIP(hipMalloc(&ptr, std::max<int64_t>(count, 1) * sizeof(T)));
+        allocs.push_back(ptr);
+        return (T*)ptr;
+    }
+    ~wcsph_sim() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (void* a : allocs) (void)hipFree(a);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : tev) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    int64_t nb(int64_t items) const { return ceil_div(items, kBlock); }
+};
+
+namespace {
+
+void check_params(const wcsph_params* p) {
+    WCSPH_REQUIRE(p != nullptr, WCSPH_EINVAL, "null params");
+    WCSPH_REQUIRE(p->dim == 2 || p->dim == 3, WCSPH_EINVAL, "dim must be 2 or 3");
+    WCSPH_REQUIRE(p->rho0 > 0 && p->c0 > 0 && p->gamma > 0 && p->c_cfl > 0 && p->eps >= 0, WCSPH_EINVAL,
+                  "rho0, c0, gamma, c_cfl must be positive and eps non-negative");
+}
+
+int64_t sim_bytes(int D, int64_t n, int K, int64_t ncell_guess) {
+    const int64_t master = (3 * D * 8 + 3 * 8 + 1 + 4) * n;  // pos vel vh, dens mass h, type, id
+    const int64_t per = 2 * master + (D * 8 + 16) * n + (4 + 16) * n + 4 * 3 * n + (int64_t)K * 4 * n + 4 * n;
+    return per + 8 * (ncell_guess + 1) + (1 << 20);
+}
+
+}  // namespace
+
+static void build_grid(wcsph_sim* s, const double* pos, const double* h) {
+    const int D = s->D;
+    const int64_t n = s->n;
+    double hmax = 0.0;
+    for (int64_t i = 0; i < n; ++i) hmax = std::max(hmax, (double)(float)h[i]);
+    WCSPH_REQUIRE(hmax > 0.0, WCSPH_EINVAL, "smoothing lengths must be positive");
+    // cell edge slightly above the largest support 2 h_max, so any pair that
+    // passes the FP32 predicate lies in adjacent cells
+    const double L = 2.0 * hmax * (1.0 + 1.0e-5);
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    bool have_box = true;
+    for (int d = 0; d < D; ++d) have_box = have_box && s->cfg.box_hi[d] > s->cfg.box_lo[d];
+    for (int d = 0; d < D; ++d) {
+        if (have_box) {
+            lo[d] = s->cfg.box_lo[d];
+            hi[d] = s->cfg.box_hi[d];
+        } else {
+            lo[d] = hi[d] = pos[d * n];
+            for (int64_t i = 1; i < n; ++i) {
+                lo[d] = std::min(lo[d], pos[d * n + i]);
+                hi[d] = std::max(hi[d], pos[d * n + i]);
+            }
+            const double ext = hi[d] - lo[d];
+            lo[d] -= 2.0 * L;
+            hi[d] += 2.0 * L + (d == D - 1 ? 0.5 * ext : 0.0);  // room above for splashes
+        }
+    }
+    s->grid.invL = 1.0 / L;
+    int64_t ncell = 1;
+    for (int d = 0; d < 3; ++d) {
+        s->grid.lo[d] = d < D ? lo[d] : 0.0;
+        s->grid.nc[d] = d < D ? (int)std::max<int64_t>(1, (int64_t)std::ceil((hi[d] - lo[d]) / L)) : 1;
+        ncell *= s->grid.nc[d];
+    }
+    WCSPH_REQUIRE(ncell < (int64_t)1 << 31, WCSPH_EOVERFLOW, "cell grid exceeds 2^31 cells");
+    s->ncell = ncell;
+    s->npart = ceil_div(ncell, kScanTile);
+    s->count = s->alloc<uint32_t>(ncell);
+    s->start = s->alloc<uint32_t>(ncell + 1);
+    s->part = s->alloc<uint32_t>(s->npart);
+}
+
+static void bin_and_reorder(wcsph_sim* s) {
+    const int64_t n = s->n;
+    Master& src = s->m[s->cur];
+    Master& dst = s->m[1 - s->cur];
+    hipLaunchKernelGGL(k_fill_u32, dim3(s->nb(s->ncell)), dim3(kBlock), 0, s->st, s->count, s->ncell, 0u);
+    if (s->D == 3)
+        hipLaunchKernelGGL(k_bin<3>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src.pos, n, s->grid, s->key, s->slot,
+                           s->count, s->sc);
+    else
+        hipLaunchKernelGGL(k_bin<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src.pos, n, s->grid, s->key, s->slot,
+                           s->count, s->sc);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s->st, s->part, s->npart);
+    hipLaunchKernelGGL(k_scan_final, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part,
+                       s->start);
+    hipLaunchKernelGGL(k_scatter, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->key, s->slot, s->start, s->perm, n);
+    hipLaunchKernelGGL(k_cell_sort, dim3(s->nb(s->ncell)), dim3(kBlock), 0, s->st, s->start, s->perm, s->ncell);
+    const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
+    EosConsts e{s->p.rho0, B, s->p.gamma};
+    if (s->D == 3)
+        hipLaunchKernelGGL(k_reorder<3>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, src, dst, s->hf, s->q4, s->pr64,
+                           s->c64, e, n);
+    else
+        hipLaunchKernelGGL(k_reorder<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, src, dst, s->hf, s->q4, s->pr64,
+                           s->c64, e, n);
+    s->cur = 1 - s->cur;
+}
+
+template <int D, typename T>
+static void launch_force(wcsph_sim* s) {
+    ForceArgs<T> a;
+    const Master& m = s->m[s->cur];
+    a.pos = m.pos;
+    a.vel = m.vel;
+    a.hf = s->hf;
+    a.q4 = s->q4;
+    a.h64 = m.h;
+    a.rho64 = m.dens;
+    a.m64 = m.mass;
+    a.pr64 = s->pr64;
+    a.c64 = s->c64;
+    a.nbr = s->nbr;
+    a.cnt = s->cnt;
+    a.acc = s->acc;
+    a.divv = s->divv;
+    a.dti = s->dti;
+    a.n = s->n;
+    a.K = s->K;
+    a.nu = s->p.nu;
+    a.c_cfl = s->p.c_cfl;
+    a.k.alpha = (T)s->p.alpha;
+    a.k.beta = (T)s->p.beta;
+    a.k.eps = (T)s->p.eps;
+    a.k.cnorm = (T)(D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI);
+    a.k.vsig_mode = s->p.vsig_mode;
+    hipLaunchKernelGGL((k_force<D, T>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
+}
+
+// e_before / e_mid / e_after (any may be null) bracket the two kernels.
+static void neigh_force(wcsph_sim* s, hipEvent_t e_before, hipEvent_t e_mid, hipEvent_t e_after) {
+    const Master& m = s->m[s->cur];
+    if (e_before) WCSPH_HIP(hipEventRecord(e_before, s->st));
+    if (s->D == 3)
+        hipLaunchKernelGGL(k_neigh<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m.pos, s->hf, s->start, s->grid,
+                           s->n, s->K, s->nbr, s->cnt, s->sc);
+    else
+        hipLaunchKernelGGL(k_neigh<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m.pos, s->hf, s->start, s->grid,
+                           s->n, s->K, s->nbr, s->cnt, s->sc);
+    if (e_mid) WCSPH_HIP(hipEventRecord(e_mid, s->st));
+    const int prec = s->cfg.precision;
+    if (prec == WCSPH_PREC_FP64) {
+        if (s->D == 3) launch_force<3, double>(s);
+        else launch_force<2, double>(s);
+    } else {
+        if (s->D == 3) launch_force<3, float>(s);
+        else launch_force<2, float>(s);
+    }
+    if (e_after) WCSPH_HIP(hipEventRecord(e_after, s->st));
+    s->forced = true;
+}
+
+static void final_stage(wcsph_sim* s, int kick) {
+    const Master& m = s->m[s->cur];
+    if (s->D == 3)
+        hipLaunchKernelGGL(k_final<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->dti, s->p.g,
+                           s->p.nu, s->p.rho_floor, kick, s->sc, s->n);
+    else
+        hipLaunchKernelGGL(k_final<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->dti, s->p.g,
+                           s->p.nu, s->p.rho_floor, kick, s->sc, s->n);
+}
+
+static hipEvent_t* step_events(wcsph_sim* s) {
+    const size_t need = (size_t)(s->tsteps + 1) * 5;
+    while (s->tev.size() < need) {
+        hipEvent_t e;
+        WCSPH_HIP(hipEventCreate(&e));
+        s->tev.push_back(e);
+    }
+    return &s->tev[(size_t)s->tsteps * 5];
+}
+
+static void one_step(wcsph_sim* s) {
+    hipEvent_t* ev = s->timing ? step_events(s) : nullptr;
+    if (ev) WCSPH_HIP(hipEventRecord(ev[0], s->st));
+    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, s->st, s->sc, 1);
+    const Master& m = s->m[s->cur];
+    if (s->D == 3)
+        hipLaunchKernelGGL(k_kick_drift<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->divv, s->sc,
+                           s->n);
+    else
+        hipLaunchKernelGGL(k_kick_drift<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->divv, s->sc,
+                           s->n);
+    bin_and_reorder(s);
+    if (ev) neigh_force(s, ev[1], ev[2], ev[3]);
+    else neigh_force(s, nullptr, nullptr, nullptr);
+    final_stage(s, 1);
+    if (ev) {
+        WCSPH_HIP(hipEventRecord(ev[4], s->st));
+        ++s->tsteps;
+    }
+}
+
+// FP64 diagnostics: per-block fixed-order tree sums, then one ordered pass.
+constexpr int kDiag = 9;
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_diag(Master m, int64_t n, double g, double rho0, double B,
+                                                 double gamma, double zfront, double xcol, double* part) {
+    __shared__ double sh[kDiag][kBlock];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[kDiag];
+    for (int k = 0; k < kDiag; ++k) v[k] = 0.0;
+    v[7] = -1.0e300;
+    v[8] = -1.0e300;
+    if (i < n) {
+        const double mass = m.mass[i];
+        double v2 = 0.0;
+        v[0] = mass;
+        for (int d = 0; d < D; ++d) {
+            const double u = m.vel[d * n + i];
+            v[1 + d] = mass * u;
+            v2 += u * u;
+        }
+        v[4] = 0.5 * mass * v2;
+        const double z = m.pos[(D - 1) * n + i];
+        v[5] = mass * g * z;
+        const double r = m.dens[i] / rho0;
+        v[6] = mass * B / rho0 * (pow(r, gamma - 1.0) / (gamma - 1.0) + 1.0 / r - gamma / (gamma - 1.0));
+        if (m.type[i] == 0) {
+            const double x = m.pos[i];
+            if (z < zfront) v[7] = x;
+            if (x < xcol) v[8] = z;
+        }
+    }
+    for (int k = 0; k < kDiag; ++k) sh[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            for (int k = 0; k < 7; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+            for (int k = 7; k < 9; ++k) sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kDiag) part[(int64_t)blockIdx.x * kDiag + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_diag_final(const double* part, int64_t nblk, double* out) {
+    if (threadIdx.x >= kDiag) return;
+    const int k = threadIdx.x;
+    double acc = k < 7 ? 0.0 : -1.0e300;
+    for (int64_t b = 0; b < nblk; ++b) {
+        const double v = part[b * kDiag + k];
+        acc = k < 7 ? acc + v : fmax(acc, v);
+    }
+    out[k] = acc;
+}
+
+extern "C" {
+
+int64_t wcsph_sim_bytes(int32_t dim, int64_t n, const wcsph_sim_config* cfg) {
+    const int K = (cfg && cfg->max_neighbours > 0) ? cfg->max_neighbours : 128;
+    return sim_bytes(dim, n, K, n / 8);
+}
+
+int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t n, wcsph_sim** out) {
+    return guarded([&] {
+        check_params(p);
+        WCSPH_REQUIRE(out != nullptr, WCSPH_EINVAL, "null output handle");
+        WCSPH_REQUIRE(n > 0 && n < ((int64_t)1 << 31), WCSPH_EINVAL, "n must be in [1, 2^31)");
+        *out = nullptr;
+        auto s = std::make_unique<wcsph_sim>();
+        s->p = *p;
+        if (cfg) s->cfg = *cfg;
+        WCSPH_REQUIRE(s->cfg.precision == WCSPH_PREC_FP32 || s->cfg.precision == WCSPH_PREC_FP64 ||
+                          s->cfg.precision == WCSPH_PREC_MIXED,
+                      WCSPH_EINVAL, "unknown precision");
+        WCSPH_REQUIRE(s->cfg.precision != WCSPH_PREC_MIXED, WCSPH_EINVAL,
+                      "mixed precision force path is not available in this build");
+        s->D = p->dim;
+        s->n = n;
+        s->K = s->cfg.max_neighbours > 0 ? s->cfg.max_neighbours : 128;
+        WCSPH_REQUIRE(s->K <= 4096, WCSPH_EINVAL, "max_neighbours above 4096");
+        s->device = p->device;
+        WCSPH_HIP(hipSetDevice(s->device));
+        WCSPH_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        for (auto& e : s->ev) WCSPH_HIP(hipEventCreate(&e));
+        const int D = s->D;
+        for (int b = 0; b < 2; ++b) {
+            Master& m = s->m[b];
+            m.pos = s->alloc<double>(D * n);
+            m.vel = s->alloc<double>(D * n);
+            m.vh = s->alloc<double>(D * n);
+            m.dens = s->alloc<double>(n);
+            m.mass = s->alloc<double>(n);
+            m.h = s->alloc<double>(n);
+            m.type = s->alloc<uint8_t>(n);
+            m.id = s->alloc<uint32_t>(n);
+        }
+        s->acc = s->alloc<double>(D * n);
+        s->divv = s->alloc<double>(n);
+        s->dti = s->alloc<double>(n);
+        s->hf = s->alloc<float>(n);
+        s->q4 = s->alloc<float4>(n);
+        if (s->cfg.precision == WCSPH_PREC_FP64) {
+            s->pr64 = s->alloc<double>(n);
+            s->c64 = s->alloc<double>(n);
+        }
+        s->key = s->alloc<uint32_t>(n);
+        s->slot = s->alloc<uint32_t>(n);
+        s->perm = s->alloc<uint32_t>(n);
+        s->nbr = s->alloc<uint32_t>((int64_t)s->K * n);
+        s->cnt = s->alloc<uint32_t>(n);
+        s->sc = s->alloc<Scalars>(1);
+        s->diag_part = s->alloc<double>(s->nb(n) * kDiag);
+        s->diag_out = s->alloc<double>(kDiag);
+        WCSPH_HIP(hipMemsetAsync(s->acc, 0, sizeof(double) * D * n, s->st));
+        WCSPH_HIP(hipMemsetAsync(s->divv, 0, sizeof(double) * n, s->st));
+        WCSPH_HIP(hipMemsetAsync(s->sc, 0, sizeof(Scalars), s->st));
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+        *out = s.release();
+    });
+}
+
+int wcsph_sim_destroy(wcsph_sim* s) {
+    return guarded([&] { delete s; });
+}
+
+int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const double* dens, const double* mass,
+                     const double* h, const int32_t* type) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && pos && vel && dens && mass && h && type, WCSPH_EINVAL, "null argument");
+        WCSPH_REQUIRE(!s->uploaded, WCSPH_ESTATE, "state already uploaded");
+        WCSPH_HIP(hipSetDevice(s->device));
+        const int64_t n = s->n;
+        const int D = s->D;
+        for (int64_t i = 0; i < n; ++i) {
+            WCSPH_REQUIRE(dens[i] > 0 && mass[i] > 0 && h[i] > 0, WCSPH_EINVAL,
+                          "density, mass and smoothing length must be positive");
+            WCSPH_REQUIRE(type[i] == 0 || type[i] == 1, WCSPH_EINVAL, "type must be 0 (fluid) or 1 (wall)");
+        }
+        build_grid(s, pos, h);
+        std::vector<uint8_t> t8(n);
+        std::vector<uint32_t> id(n);
+        for (int64_t i = 0; i < n; ++i) {
+            t8[i] = (uint8_t)type[i];
+            id[i] = (uint32_t)i;
+        }
+        const Master& m = s->m[s->cur];
+        WCSPH_HIP(hipMemcpy(m.pos, pos, sizeof(double) * D * n, hipMemcpyHostToDevice));
+        WCSPH_HIP(hipMemcpy(m.vel, vel, sizeof(double) * D * n, hipMemcpyHostToDevice));
+        WCSPH_HIP(hipMemcpy(m.vh, vel, sizeof(double) * D * n, hipMemcpyHostToDevice));
+        WCSPH_HIP(hipMemcpy(m.dens, dens, sizeof(double) * n, hipMemcpyHostToDevice));
+        WCSPH_HIP(hipMemcpy(m.mass, mass, sizeof(double) * n, hipMemcpyHostToDevice));
+        WCSPH_HIP(hipMemcpy(m.h, h, sizeof(double) * n, hipMemcpyHostToDevice));
+        WCSPH_HIP(hipMemcpy(m.type, t8.data(), n, hipMemcpyHostToDevice));
+        WCSPH_HIP(hipMemcpy(m.id, id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+        s->uploaded = true;
+    });
+}
+
+int wcsph_sim_init(wcsph_sim* s) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "upload a state first");
+        WCSPH_HIP(hipSetDevice(s->device));
+        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, s->st, s->sc, 0);
+        bin_and_reorder(s);
+        neigh_force(s, nullptr, nullptr, nullptr);
+        final_stage(s, 0);
+        WCSPH_HIP(hipGetLastError());
+    });
+}
+
+int wcsph_sim_step(wcsph_sim* s, int32_t nsteps) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && s->uploaded && s->forced, WCSPH_ESTATE, "call wcsph_sim_init first");
+        WCSPH_REQUIRE(nsteps >= 0, WCSPH_EINVAL, "nsteps < 0");
+        WCSPH_HIP(hipSetDevice(s->device));
+        for (int k = 0; k < nsteps; ++k) one_step(s);
+        WCSPH_HIP(hipGetLastError());
+    });
+}
+
+int wcsph_sim_force(wcsph_sim* s) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "upload a state first");
+        WCSPH_HIP(hipSetDevice(s->device));
+        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, s->st, s->sc, 2);
+        WCSPH_HIP(hipEventRecord(s->ev[0], s->st));
+        bin_and_reorder(s);
+        neigh_force(s, s->ev[1], s->ev[2], s->ev[3]);
+        WCSPH_HIP(hipGetLastError());
+    });
+}
+
+int wcsph_sim_last_force_ms(wcsph_sim* s, float* out) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && out, WCSPH_EINVAL, "null argument");
+        WCSPH_HIP(hipEventSynchronize(s->ev[3]));
+        WCSPH_HIP(hipEventElapsedTime(&out[0], s->ev[0], s->ev[1]));
+        WCSPH_HIP(hipEventElapsedTime(&out[1], s->ev[1], s->ev[2]));
+        WCSPH_HIP(hipEventElapsedTime(&out[2], s->ev[2], s->ev[3]));
+    });
+}
+
+int wcsph_sim_set_timing(wcsph_sim* s, int32_t enable) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s, WCSPH_EINVAL, "null simulation");
+        s->timing = enable != 0;
+        s->tsteps = 0;
+    });
+}
+
+int wcsph_sim_timing(wcsph_sim* s, double* out, int64_t* nsteps) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && out, WCSPH_EINVAL, "null argument");
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+        double acc[5] = {0, 0, 0, 0, 0};
+        for (int64_t k = 0; k < s->tsteps; ++k) {
+            hipEvent_t* e = &s->tev[(size_t)k * 5];
+            float ms;
+            for (int q = 0; q < 4; ++q) {
+                WCSPH_HIP(hipEventElapsedTime(&ms, e[q], e[q + 1]));
+                acc[q] += ms;
+            }
+            WCSPH_HIP(hipEventElapsedTime(&ms, e[0], e[4]));
+            acc[4] += ms;
+        }
+        for (int q = 0; q < 5; ++q) out[q] = acc[q];
+        if (nsteps) *nsteps = s->tsteps;
+        s->tsteps = 0;
+    });
+}
+
+int wcsph_sim_sync(wcsph_sim* s) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s, WCSPH_EINVAL, "null simulation");
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+    });
+}
+
+int wcsph_sim_get_stats(wcsph_sim* s, wcsph_sim_stats* st) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && st, WCSPH_EINVAL, "null argument");
+        Scalars h{};
+        WCSPH_HIP(hipMemcpyAsync(&h, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, s->st));
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+        st->time = h.time;
+        st->dt = h.dt;
+        st->step = h.step;
+        st->n = s->n;
+        st->pairs_last = (int64_t)h.pairs_last;
+        st->pairs_total = (int64_t)h.pairs_total;
+        st->escaped = h.escaped;
+        st->max_count = (int32_t)h.maxcnt;
+        st->overflow = (int32_t)h.overflow;
+    });
+}
+
+int wcsph_sim_download(wcsph_sim* s, double* pos, double* vel, double* dens, double* acc, double* div_v,
+                       double* dt_i) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "nothing uploaded");
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+        const int64_t n = s->n;
+        const int D = s->D;
+        const Master& m = s->m[s->cur];
+        std::vector<uint32_t> id(n);
+        WCSPH_HIP(hipMemcpy(id.data(), m.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        std::vector<double> tmp((size_t)D * n);
+        auto fetch = [&](const double* src, double* dst, int comps) {
+            if (!dst) return;
+            WCSPH_HIP(hipMemcpy(tmp.data(), src, sizeof(double) * comps * n, hipMemcpyDeviceToHost));
+            for (int c = 0; c < comps; ++c)
+                for (int64_t p = 0; p < n; ++p) dst[c * n + id[p]] = tmp[c * n + p];
+        };
+        fetch(m.pos, pos, D);
+        fetch(m.vel, vel, D);
+        fetch(m.dens, dens, 1);
+        fetch(s->acc, acc, D);
+        fetch(s->divv, div_v, 1);
+        fetch(s->dti, dt_i, 1);
+    });
+}
+
+int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t cap) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && s->forced && counts, WCSPH_ESTATE, "no force pass yet or null counts");
+        WCSPH_REQUIRE(!lists || cap > 0, WCSPH_EINVAL, "cap must be positive with lists");
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+        const int64_t n = s->n;
+        const Master& m = s->m[s->cur];
+        std::vector<uint32_t> id(n), c(n);
+        WCSPH_HIP(hipMemcpy(id.data(), m.id, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        WCSPH_HIP(hipMemcpy(c.data(), s->cnt, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        for (int64_t p = 0; p < n; ++p) counts[id[p]] = (int32_t)c[p];
+        if (!lists) return;
+        std::vector<uint32_t> nb((size_t)s->K * n);
+        WCSPH_HIP(hipMemcpy(nb.data(), s->nbr, sizeof(uint32_t) * s->K * n, hipMemcpyDeviceToHost));
+        std::vector<int32_t> row;
+        for (int64_t p = 0; p < n; ++p) {
+            const int k = (int)std::min<uint32_t>(c[p], (uint32_t)s->K);
+            row.resize(k);
+            for (int t = 0; t < k; ++t) row[t] = (int32_t)id[nb[(size_t)t * n + p]];
+            std::sort(row.begin(), row.end());
+            int32_t* out = lists + (int64_t)id[p] * cap;
+            for (int t = 0; t < cap; ++t) out[t] = t < k ? row[t] : -1;
+        }
+    });
+}
+
+int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double* out) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && s->uploaded && out, WCSPH_EINVAL, "null argument or nothing uploaded");
+        WCSPH_HIP(hipSetDevice(s->device));
+        const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
+        const int64_t nblk = s->nb(s->n);
+        const Master& m = s->m[s->cur];
+        if (s->D == 3)
+            hipLaunchKernelGGL(k_diag<3>, dim3(nblk), dim3(kBlock), 0, s->st, m, s->n, s->p.g, s->p.rho0, B,
+                               s->p.gamma, zfront, xcol, s->diag_part);
+        else
+            hipLaunchKernelGGL(k_diag<2>, dim3(nblk), dim3(kBlock), 0, s->st, m, s->n, s->p.g, s->p.rho0, B,
+                               s->p.gamma, zfront, xcol, s->diag_part);
+        hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->st, s->diag_part, nblk, s->diag_out);
+        WCSPH_HIP(hipGetLastError());
+        WCSPH_HIP(hipMemcpyAsync(out, s->diag_out, sizeof(double) * kDiag, hipMemcpyDeviceToHost, s->st));
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+    });
+}
+
+}  // extern "C"
