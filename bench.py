@@ -68,6 +68,8 @@ def main():
     ap.add_argument("--dx", type=float, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cells-per-support", type=int, default=0, help="1 or 2 (0: library default)")
+    ap.add_argument("--queue-depth", type=int, default=0, help="16 or 32 (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -84,7 +86,8 @@ def main():
     n = len(st["dens"])
     print(f"# {dim}D dam break dx={dx}: {st['n_fluid']} fluid + {st['n_wall']} wall = {n} particles, "
           f"precision {args.precision}", file=sys.stderr, flush=True)
-    sim = W.Simulation.from_state(params, st, precision=args.precision)
+    sim = W.Simulation.from_state(params, st, precision=args.precision, cells_per_support=args.cells_per_support,
+                                  queue_depth=args.queue_depth)
     sim.initial_state = st
     sim.init()
     t0 = time.perf_counter()
@@ -124,7 +127,9 @@ def main():
         "data": "synthetic (dam-break lattice initial condition, developed by the warmup steps)",
         "config": {"workload": f"{dim}D dam break, dx={dx} m, {n} particles ({st['n_fluid']} fluid), "
                                f"{args.precision} force path, full device-resident steps",
-                   "particles": n, "dx": dx, "precision": args.precision, "step_window":
+                   "particles": n, "dx": dx, "precision": args.precision,
+                   "cells_per_support": args.cells_per_support or 2, "queue_depth": args.queue_depth or 16,
+                   "step_window":
                        f"steps {args.warmup + 1}-{args.warmup + K}"},
         "steps_per_s": K / wall,
         "pairs_per_step": pairs_launch,
@@ -137,6 +142,7 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "flop_per_pair": FLOP_PER_PAIR[dim], "avg_launch_ms": force_ms},
     }
+    line["hbm_copy_GBps"] = W.copy_probe_gbps(1 << 30, 10)
     if rank == 0 and args.cpu_seconds > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"# cpu baseline on {threads} threads", file=sys.stderr, flush=True)

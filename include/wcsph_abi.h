@@ -72,6 +72,10 @@ int wcsph_params_reference(int32_t dim, wcsph_params* p);
 const char* wcsph_last_error(void);
 /* Library version string, for logs. */
 const char* wcsph_version(void);
+/* HBM bandwidth probe: `reps` streaming copies of `bytes` on `device`;
+ * *gbps = (read + write bytes) / time. Used by the bench for the measured
+ * HBM peak and to calibrate hardware byte counters (kernel k_copy). */
+int wcsph_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps);
 
 /* ---------------- 1. FDPS multiwalk drop-in ---------------- */
 /* Replaces the lazy first-call set-up of src/kernel.cu:290-305. */
@@ -104,9 +108,12 @@ typedef struct wcsph_sim wcsph_sim;
 
 typedef struct {
     int32_t precision;      /* WCSPH_PREC_* */
-    int32_t max_neighbours; /* neighbour list capacity per particle (0: 128) */
+    int32_t max_neighbours; /* list capacity per particle for the FP64 path and
+                               wcsph_sim_neighbours (0: 128) */
     double box_lo[3];       /* cell-grid domain; particles outside are clamped */
     double box_hi[3];       /*   into the edge cells and counted as escaped */
+    int32_t cells_per_support; /* cell edge = largest support / this (1 or 2; 0: 2) */
+    int32_t queue_depth;    /* fused pass: LDS queue slots per lane (16 or 32; 0: 16) */
 } wcsph_sim_config;
 
 typedef struct {
@@ -168,8 +175,12 @@ int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t 
  * out[7] dam front (max x of fluid with vertical coordinate < zfront),
  * out[8] column height (max vertical coordinate of fluid with x < xcol). */
 int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double* out);
+/* The cell grid (for checking the neighbour predicate, which is defined
+ * on cell-local coordinates): lo[3], cell edge *L, cells per axis nc[3]. */
+int wcsph_sim_grid(wcsph_sim* s, double* lo, double* L, int32_t* nc);
 /* Device time (ms) of the kernels of the last wcsph_sim_force call:
- * out[0] binning, out[1] neighbour pass, out[2] force pass. */
+ * out[0] binning + reorder, out[1] separate neighbour pass (FP64 path;
+ * 0 when fused), out[2] force pass. */
 int wcsph_sim_last_force_ms(wcsph_sim* s, float* out);
 /* Per-step HIP-event timing on the simulation's stream. While enabled, each
  * step records events around its stages; wcsph_sim_timing waits, returns the

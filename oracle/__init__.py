@@ -70,6 +70,9 @@ def lib():
         L.orc_nbr_pred.restype = C.c_int
         L.orc_neighbours_brute.restype = C.c_int
         L.orc_neighbours_brute.argtypes = [C.c_int, C.c_int64, f64p, f32p, i32p, C.c_void_p, C.c_int]
+        L.orc_neighbours_brute_cells.restype = C.c_int
+        L.orc_neighbours_brute_cells.argtypes = [C.c_int, C.c_int64, f64p, f32p, f64p, C.c_double, i32p, i32p,
+                                                 C.c_void_p, C.c_int]
         L.orc_make_walks.restype = C.c_int64
         L.orc_make_walks.argtypes = [C.c_int, C.c_int64, f64p, C.c_double, C.c_int,
                                      C.POINTER(C.c_int64), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -160,6 +163,18 @@ def neighbours_brute(pos, h, cap=0):
     lib().orc_neighbours_brute(dim, n, np.ascontiguousarray(pos, np.float64),
                                np.ascontiguousarray(h, np.float32), counts,
                                lists.ctypes.data if cap else None, cap)
+    return (counts, lists) if cap else counts
+
+
+def neighbours_brute_cells(pos, h, lo, L, nc, cap=0):
+    """Brute force with the cell-local FP32 predicate of the device path."""
+    dim, n = pos.shape
+    counts = np.zeros(n, np.int32)
+    lists = np.full((n, cap), -1, np.int32) if cap else None
+    lib().orc_neighbours_brute_cells(dim, n, np.ascontiguousarray(pos, np.float64),
+                                     np.ascontiguousarray(h, np.float32), np.ascontiguousarray(lo[:dim], np.float64),
+                                     float(L), np.ascontiguousarray(nc[:dim], np.int32), counts,
+                                     lists.ctypes.data if cap else None, cap)
     return (counts, lists) if cap else counts
 
 

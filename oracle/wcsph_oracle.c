@@ -198,6 +198,60 @@ int orc_neighbours_brute(int dim, int64_t n, const double* pos, const float* h,
     return mx;
 }
 
+static void cell_local(int dim, const double* x, const double* lo, double L, const int* nc,
+                       int* c, float* xl) {
+    for (int k = 0; k < dim; ++k) {
+        const double u = (x[k] - lo[k]) * (1.0 / L);
+        int ck = (int)floor(u);
+        if (!(u >= 0.0)) ck = 0;
+        if (ck > nc[k] - 1) ck = nc[k] - 1;
+        const double cl = (double)ck * L;
+        const double origin = lo[k] + cl;
+        c[k] = ck;
+        xl[k] = (float)(x[k] - origin);
+    }
+}
+
+int orc_neighbours_brute_cells(int dim, int64_t n, const double* pos, const float* h,
+                               const double* lo, double L, const int* nc,
+                               int* counts, int* lists, int cap) {
+    int* ci = (int*)malloc(sizeof(int) * 3 * (size_t)n);
+    float* xl = (float*)malloc(sizeof(float) * 3 * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        double x[3];
+        gather_pos(dim, n, pos, i, x);
+        cell_local(dim, x, lo, L, nc, ci + 3 * i, xl + 3 * i);
+    }
+    const float Lf = (float)L;
+    int mx = 0;
+#pragma omp parallel for schedule(static) reduction(max : mx)
+    for (int64_t i = 0; i < n; ++i) {
+        int c = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            float r2 = 0.0f;
+            for (int k = 0; k < dim; ++k) {
+                const float t = xl[3 * i + k] - xl[3 * j + k];
+                const float off = (float)(ci[3 * i + k] - ci[3 * j + k]) * Lf;
+                const float d = t + off;
+                const float d2 = d * d;
+                r2 = r2 + d2;
+            }
+            const float H = h[i] + h[j];
+            const float H2 = H * H;
+            if (r2 > 0.0f && r2 < H2) {
+                if (lists && c < cap) lists[i * cap + c] = (int)j;
+                ++c;
+            }
+        }
+        counts[i] = c;
+        if (c > mx) mx = c;
+    }
+    free(ci);
+    free(xl);
+    return mx;
+}
+
 /* ------------------------------------------------------------------ */
 /* Fake FDPS multiwalk grouping                                        */
 /* ------------------------------------------------------------------ */

@@ -69,6 +69,17 @@ int orc_nbr_pred(int dim, const double* xi, const double* xj, float hi, float hj
 int orc_neighbours_brute(int dim, int64_t n, const double* pos, const float* h,
                          int* counts, int* lists, int cap);
 
+/* The device-resident path's neighbour predicate, defined on a cell grid
+ * (origin lo, cell edge L, nc cells per axis; cells of size L >= support):
+ * c = clamp(floor((x - lo) / L)), origin = lo + c L (FP64, no contraction),
+ * xl = (float)(x - origin); for a pair, d = (xl_i - xl_j) + (float)(c_i - c_j)
+ * * (float)L per component (FP32), r2 = sum of d*d (FP32, no contraction);
+ * neighbour iff r2 > 0 && r2 < (h_i + h_j)^2 (FP32). Same outputs as
+ * orc_neighbours_brute. */
+int orc_neighbours_brute_cells(int dim, int64_t n, const double* pos, const float* h,
+                               const double* lo, double L, const int* nc,
+                               int* counts, int* lists, int cap);
+
 /* Fake FDPS multiwalk grouping: Morton-ordered i-groups of at most
  * group_max particles, j list = every particle within r_search of the
  * group's bounding box (a superset of each i's support, as FDPS Symmetry

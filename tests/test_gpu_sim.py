@@ -33,31 +33,44 @@ def rel(a, b):
 SMALL3 = dict(tank_l=2.0, col_h=1.0, tank_h=1.5)
 
 
+def oracle_pairs(O, sim, s):
+    lo, L, nc = sim.grid()
+    return O.neighbours_brute_cells(s.x, s.h.astype(np.float32), lo, L, nc)
+
+
+@pytest.mark.parametrize("cps", [1, 2])
 @pytest.mark.parametrize("dim,dx,geom", [(2, 0.02, {}), (3, 0.04, SMALL3)])
-def test_neighbour_sets_bit_exact(O, W, dim, dx, geom):
+def test_neighbour_sets_bit_exact(O, W, dim, dx, geom, cps):
     s, p, _ = developed(O, W, dim, dx, 25, **geom)
     # jitter so many pairs sit near the support boundary
     rng = np.random.default_rng(11)
     s.x[:] += rng.uniform(-0.3, 0.3, s.x.shape) * dx
-    with W.Simulation.from_state(p, state_dict(s)) as sim:
+    with W.Simulation.from_state(p, state_dict(s), cells_per_support=cps) as sim:
         sim.force()
-        cnt, lists = sim.neighbours(cap=128)
         st = sim.stats()
-    rc, rl = O.neighbours_brute(s.x, s.h.astype(np.float32), cap=128)
+        cnt, lists = sim.neighbours(cap=128)
+        lo, L, nc = sim.grid()
+    rc, rl = O.neighbours_brute_cells(s.x, s.h.astype(np.float32), lo, L, nc, cap=128)
     assert np.array_equal(cnt, rc)
-    assert np.array_equal(lists, np.where(np.arange(128)[None, :] < rc[:, None], rl, -1))
+    srt = np.sort(np.where(rl >= 0, rl, np.iinfo(np.int32).max), axis=1)
+    srt = np.where(srt == np.iinfo(np.int32).max, -1, srt)
+    assert np.array_equal(lists, srt)
+    # the fused force pass evaluated exactly these pairs
     assert st["pairs_last"] == int(rc.sum()) and st["overflow"] == 0 and st["escaped"] == 0
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp64", 1e-12)])
+@pytest.mark.parametrize("precision,tol,cps,queue", [("fp32", 1e-5, 2, 16), ("fp32", 1e-5, 1, 32),
+                                                     ("fp32", 1e-5, 2, 32), ("fp64", 1e-12, 2, 0)])
 @pytest.mark.parametrize("dim,dx,geom", [(2, 0.02, {}), (3, 0.04, SMALL3)])
-def test_force_pass_matches_oracle(O, W, dim, dx, geom, precision, tol):
+def test_force_pass_matches_oracle(O, W, dim, dx, geom, precision, tol, cps, queue):
     s, p, op = developed(O, W, dim, dx, 40, **geom)
-    pairs = s.force(op, 8)  # hydro only, on the same state
-    with W.Simulation.from_state(p, state_dict(s), precision=precision) as sim:
+    s.force(op, 8)  # hydro only, on the same state
+    with W.Simulation.from_state(p, state_dict(s), precision=precision, cells_per_support=cps,
+                                 queue_depth=queue) as sim:
         sim.force()
         out = sim.download()
         st = sim.stats()
+        pairs = int(oracle_pairs(O, sim, s).sum())
     assert rel(out["acc"], s.a) <= tol
     assert rel(out["div_v"], s.divv) <= tol
     assert np.allclose(out["dt"], s.dt_i, rtol=1e-6 if precision == "fp32" else 1e-13)

@@ -404,3 +404,28 @@ def test_step_composition(O):
     assert np.allclose(s.a, acc, rtol=1e-10, atol=1e-10 * np.abs(acc).max())
     assert np.allclose(s.v, v1, rtol=1e-10, atol=1e-12)
     assert np.allclose(s.dens, rho1, rtol=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cell_local_predicate(O, dim):
+    # the device path's predicate on cell-local FP32 coordinates agrees with
+    # the FP64 distance up to rounding at the support boundary, is symmetric,
+    # and does not depend on where the grid origin sits
+    n = 700
+    pos = RNG.uniform(0, 1, (dim, n))
+    h = (0.05 * (1 + 0.3 * RNG.uniform(-1, 1, n))).astype(np.float32)
+    L = 2 * float(h.max()) * (1 + 1e-5) / 2
+    lo = np.full(3, -0.1)
+    nc = np.array([int(np.ceil(1.2 / L))] * 3, np.int32)
+    cnt, lists = O.neighbours_brute_cells(pos, h, lo, L, nc, cap=256)
+    d = np.sqrt(((pos[:, :, None] - pos[:, None, :]) ** 2).sum(0))
+    H = h[:, None].astype(np.float64) + h[None, :]
+    exact = (d < H) & (d > 0)
+    border = np.abs(d - H) < 1e-5 * H
+    got = np.zeros_like(exact)
+    for i in range(n):
+        got[i, lists[i, :cnt[i]]] = True
+    assert np.all((got == exact) | border)
+    assert np.array_equal(got, got.T)
+    cnt2 = O.neighbours_brute_cells(pos, h, lo - 0.37 * L, L, nc + 1)
+    assert np.abs(cnt2 - cnt).sum() <= max(2, int(border.sum()))

@@ -89,6 +89,13 @@ def last_kernel_ms(tag=0):
     return ms.value
 
 
+def copy_probe_gbps(nbytes=1 << 30, reps=10, device=0):
+    """Measured HBM streaming-copy bandwidth (read + write GB/s)."""
+    v = C.c_double()
+    check(lib().wcsph_copy_probe(device, nbytes, reps, C.byref(v)))
+    return v.value
+
+
 def last_error():
     return lib().wcsph_last_error().decode()
 
@@ -120,13 +127,16 @@ def dam_break_state(desc, params):
 class Simulation:
     """One device-resident SPH system (all particle data in HBM)."""
 
-    def __init__(self, params, n, *, precision="fp32", max_neighbours=128, box_lo=None, box_hi=None):
+    def __init__(self, params, n, *, precision="fp32", max_neighbours=128, box_lo=None, box_hi=None,
+                 cells_per_support=0, queue_depth=0):
         self.params = params
         self.dim = params.dim
         self.n = int(n)
         cfg = SimConfig()
         cfg.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         cfg.max_neighbours = max_neighbours
+        cfg.cells_per_support = cells_per_support
+        cfg.queue_depth = queue_depth
         if box_lo is not None:
             for k in range(self.dim):
                 cfg.box_lo[k], cfg.box_hi[k] = box_lo[k], box_hi[k]
@@ -201,6 +211,13 @@ class Simulation:
         check(lib().wcsph_sim_diagnostics(self._h, zfront, xcol, ptr(out)))
         keys = ["mass", "mom_x", "mom_y", "mom_z", "kinetic", "potential", "internal", "front", "column"]
         return dict(zip(keys, out))
+
+    def grid(self):
+        lo = np.zeros(3)
+        L = C.c_double()
+        nc = np.zeros(3, np.int32)
+        check(lib().wcsph_sim_grid(self._h, ptr(lo), C.byref(L), ptr(nc)))
+        return lo, L.value, nc
 
     def set_timing(self, enable=True):
         check(lib().wcsph_sim_set_timing(self._h, 1 if enable else 0))

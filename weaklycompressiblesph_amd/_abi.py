@@ -33,7 +33,7 @@ class Params(C.Structure):
 class SimConfig(C.Structure):
     """wcsph_sim_config."""
     _fields_ = [("precision", C.c_int32), ("max_neighbours", C.c_int32), ("box_lo", C.c_double * 3),
-                ("box_hi", C.c_double * 3)]
+                ("box_hi", C.c_double * 3), ("cells_per_support", C.c_int32), ("queue_depth", C.c_int32)]
 
 
 class SimStats(C.Structure):
@@ -69,12 +69,14 @@ def res_dtype(dim):
 
 
 EXPORTS = [
-    "wcsph_params_reference", "wcsph_last_error", "wcsph_version", "wcsph_init", "wcsph_finalize",
+    "wcsph_params_reference", "wcsph_last_error", "wcsph_version", "wcsph_copy_probe", "wcsph_init",
+    "wcsph_finalize",
     "wcsph_hydro_dispatch_3d", "wcsph_hydro_dispatch_2d", "wcsph_hydro_retrieve_3d", "wcsph_hydro_retrieve_2d",
     "wcsph_hydro_last_kernel_ms", "wcsph_dam_break_count", "wcsph_dam_break_fill", "wcsph_sim_bytes",
     "wcsph_sim_create", "wcsph_sim_destroy", "wcsph_sim_upload", "wcsph_sim_init", "wcsph_sim_step",
     "wcsph_sim_force", "wcsph_sim_sync", "wcsph_sim_get_stats", "wcsph_sim_download", "wcsph_sim_neighbours",
     "wcsph_sim_diagnostics", "wcsph_sim_last_force_ms", "wcsph_sim_set_timing", "wcsph_sim_timing",
+    "wcsph_sim_grid",
 ]
 
 _lib = None
@@ -95,6 +97,7 @@ def lib():
         "wcsph_params_reference": (C.c_int, [I32, C.POINTER(Params)]),
         "wcsph_last_error": (C.c_char_p, []),
         "wcsph_version": (C.c_char_p, []),
+        "wcsph_copy_probe": (C.c_int, [I32, I64, I32, C.POINTER(F64)]),
         "wcsph_init": (C.c_int, [C.POINTER(Params)]),
         "wcsph_finalize": (C.c_int, []),
         "wcsph_hydro_dispatch_3d": (C.c_int, [I32, I32, VP, VP, VP, VP]),
@@ -118,6 +121,7 @@ def lib():
         "wcsph_sim_diagnostics": (C.c_int, [VP, F64, F64, VP]),
         "wcsph_sim_last_force_ms": (C.c_int, [VP, VP]),
         "wcsph_sim_set_timing": (C.c_int, [VP, I32]),
+        "wcsph_sim_grid": (C.c_int, [VP, VP, VP, VP]),
         "wcsph_sim_timing": (C.c_int, [VP, VP, C.POINTER(I64)]),
     }
     for name, (res, args) in sig.items():

@@ -4,9 +4,12 @@
 //   k_scalars(commit dt)      getTimeStepGlobal result of the previous step
 //   k_kick_drift              initialKick + fullDrift + predict (FP64)
 //   k_fill/k_bin/scan/scatter/k_cell_sort   cell list by counting sort
-//   k_reorder                 permute the state into cell order + EoS
-//   k_neigh                   neighbour lists (replaces the FDPS tree walk)
-//   k_force                   hydro force (replaces dispatch/retrieve)
+//   k_reorder                 permute the state into cell order + EoS,
+//                             compact FP32 force records (cell-local x)
+//   k_nforce                  fused neighbour search + hydro force (FP32;
+//                             replaces the FDPS tree walk and
+//                             dispatch/retrieve); the FP64 path runs
+//                             k_neigh (lists) + k_force64 instead
 //   k_final                   gravity + finalKick + density floor + dt bound
 // Nothing returns to the host between steps; dt lives on the device.
 #include <cmath>
@@ -31,13 +34,17 @@ struct wcsph_sim {
     Master m[2]{};
     int cur = 0;
     double *acc = nullptr, *divv = nullptr, *dti = nullptr;
-    float* hf = nullptr;
-    float4* q4 = nullptr;
+    Packed pk{};
+    int R = 2;      // cells per support length
+    int Q = 16;     // LDS queue depth of the fused pass
+    float hmax = 0.f;
+    bool uniform_h = false;  // all smoothing lengths equal (fixed in time)
     double *pr64 = nullptr, *c64 = nullptr;  // FP64 path only
     uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr,
              *part = nullptr;
     uint32_t *nbr = nullptr, *cnt = nullptr;
     Scalars* sc = nullptr;
+    Partials P{};
     double* diag_part = nullptr;
     double* diag_out = nullptr;
     bool uploaded = false, forced = false;
@@ -75,7 +82,7 @@ void check_params(const wcsph_params* p) {
 
 int64_t sim_bytes(int D, int64_t n, int K, int64_t ncell_guess) {
     const int64_t master = (3 * D * 8 + 3 * 8 + 1 + 4) * n;  // pos vel vh, dens mass h, type, id
-    const int64_t per = 2 * master + (D * 8 + 16) * n + (4 + 16) * n + 4 * 3 * n + (int64_t)K * 4 * n + 4 * n;
+    const int64_t per = 2 * master + (D * 8 + 16) * n + 52 * n + 4 * 3 * n + (int64_t)K * 4 * n + 4 * n;
     return per + 8 * (ncell_guess + 1) + (1 << 20);
 }
 
@@ -86,10 +93,13 @@ static void build_grid(wcsph_sim* s, const double* pos, const double* h) {
     const int64_t n = s->n;
     double hmax = 0.0;
     for (int64_t i = 0; i < n; ++i) hmax = std::max(hmax, (double)(float)h[i]);
+    s->uniform_h = true;
+    for (int64_t i = 0; i < n; ++i) s->uniform_h = s->uniform_h && (float)h[i] == (float)h[0];
     WCSPH_REQUIRE(hmax > 0.0, WCSPH_EINVAL, "smoothing lengths must be positive");
     // cell edge slightly above the largest support 2 h_max, so any pair that
     // passes the FP32 predicate lies in adjacent cells
-    const double L = 2.0 * hmax * (1.0 + 1.0e-5);
+    const double L = 2.0 * hmax * (1.0 + 1.0e-5) / s->R;
+    s->hmax = (float)hmax;
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     bool have_box = true;
     for (int d = 0; d < D; ++d) have_box = have_box && s->cfg.box_hi[d] > s->cfg.box_lo[d];
@@ -109,6 +119,8 @@ static void build_grid(wcsph_sim* s, const double* pos, const double* h) {
         }
     }
     s->grid.invL = 1.0 / L;
+    s->grid.L = L;
+    s->grid.Lf = (float)L;
     int64_t ncell = 1;
     for (int d = 0; d < 3; ++d) {
         s->grid.lo[d] = d < D ? lo[d] : 0.0;
@@ -143,27 +155,57 @@ static void bin_and_reorder(wcsph_sim* s) {
     const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
     EosConsts e{s->p.rho0, B, s->p.gamma};
     if (s->D == 3)
-        hipLaunchKernelGGL(k_reorder<3>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, src, dst, s->hf, s->q4, s->pr64,
-                           s->c64, e, n);
+        hipLaunchKernelGGL(k_reorder<3>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key, src, dst, s->pk,
+                           s->grid, s->pr64, s->c64, e, n);
     else
-        hipLaunchKernelGGL(k_reorder<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, src, dst, s->hf, s->q4, s->pr64,
-                           s->c64, e, n);
+        hipLaunchKernelGGL(k_reorder<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key, src, dst, s->pk,
+                           s->grid, s->pr64, s->c64, e, n);
     s->cur = 1 - s->cur;
 }
 
-template <int D, typename T>
-static void launch_force(wcsph_sim* s) {
-    ForceArgs<T> a;
+template <class T>
+static PairConsts<T> pair_consts(const wcsph_sim* s) {
+    PairConsts<T> k;
+    k.alpha = (T)s->p.alpha;
+    k.beta = (T)s->p.beta;
+    k.eps = (T)s->p.eps;
+    k.cnorm = (T)(s->D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI);
+    k.vsig_mode = s->p.vsig_mode;
+    return k;
+}
+
+static void ensure_lists(wcsph_sim* s) {
+    if (!s->nbr) {
+        s->nbr = s->alloc<uint32_t>((int64_t)s->K * s->n);
+        s->cnt = s->alloc<uint32_t>(s->n);
+    }
+}
+
+template <int D, int R>
+static void launch_neigh(wcsph_sim* s, int count_total) {
+    ensure_lists(s);
+    hipLaunchKernelGGL((k_neigh<D, R>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, s->pk, s->start, s->grid, s->n,
+                       s->K, s->hmax, s->nbr, s->cnt, s->P);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc, count_total ? 5 : 1);
+}
+
+// count_total = 0 for list rebuilds that must not add to the pair totals
+static void neigh_lists(wcsph_sim* s, int count_total = 1) {
+    if (s->D == 3) s->R == 1 ? launch_neigh<3, 1>(s, count_total) : launch_neigh<3, 2>(s, count_total);
+    else s->R == 1 ? launch_neigh<2, 1>(s, count_total) : launch_neigh<2, 2>(s, count_total);
+}
+
+template <int D>
+static void launch_force64(wcsph_sim* s) {
     const Master& m = s->m[s->cur];
+    ForceArgs64 a;
     a.pos = m.pos;
     a.vel = m.vel;
-    a.hf = s->hf;
-    a.q4 = s->q4;
-    a.h64 = m.h;
-    a.rho64 = m.dens;
-    a.m64 = m.mass;
-    a.pr64 = s->pr64;
-    a.c64 = s->c64;
+    a.h = m.h;
+    a.rho = m.dens;
+    a.m = m.mass;
+    a.pr = s->pr64;
+    a.c = s->c64;
     a.nbr = s->nbr;
     a.cnt = s->cnt;
     a.acc = s->acc;
@@ -173,32 +215,54 @@ static void launch_force(wcsph_sim* s) {
     a.K = s->K;
     a.nu = s->p.nu;
     a.c_cfl = s->p.c_cfl;
-    a.k.alpha = (T)s->p.alpha;
-    a.k.beta = (T)s->p.beta;
-    a.k.eps = (T)s->p.eps;
-    a.k.cnorm = (T)(D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI);
-    a.k.vsig_mode = s->p.vsig_mode;
-    hipLaunchKernelGGL((k_force<D, T>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
+    a.k = pair_consts<double>(s);
+    hipLaunchKernelGGL(k_force64<D>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
 }
 
-// e_before / e_mid / e_after (any may be null) bracket the two kernels.
+template <int D, int R, int Q, bool UH>
+static void launch_nforce(wcsph_sim* s) {
+    NFArgs a;
+    a.pk = s->pk;
+    a.start = s->start;
+    a.g = s->grid;
+    a.n = s->n;
+    a.hmax = s->hmax;
+    a.nu = (float)s->p.nu;
+    a.c_cfl = s->p.c_cfl;
+    a.k = pair_consts<float>(s);
+    a.acc = s->acc;
+    a.divv = s->divv;
+    a.dti = s->dti;
+    a.P = s->P;
+    hipLaunchKernelGGL((k_nforce<D, R, Q, UH>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc, 5);
+}
+
+template <int D, bool UH>
+static void nforce_dispatch_h(wcsph_sim* s) {
+    if (s->R == 1) s->Q == 32 ? launch_nforce<D, 1, 32, UH>(s) : launch_nforce<D, 1, 16, UH>(s);
+    else s->Q == 32 ? launch_nforce<D, 2, 32, UH>(s) : launch_nforce<D, 2, 16, UH>(s);
+}
+
+template <int D>
+static void nforce_dispatch(wcsph_sim* s) {
+    if (s->uniform_h) nforce_dispatch_h<D, true>(s);
+    else nforce_dispatch_h<D, false>(s);
+}
+
+// e_before / e_mid / e_after (any may be null) bracket the neighbour and
+// force kernels (e_mid directly follows e_before on the fused path).
 static void neigh_force(wcsph_sim* s, hipEvent_t e_before, hipEvent_t e_mid, hipEvent_t e_after) {
-    const Master& m = s->m[s->cur];
     if (e_before) WCSPH_HIP(hipEventRecord(e_before, s->st));
-    if (s->D == 3)
-        hipLaunchKernelGGL(k_neigh<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m.pos, s->hf, s->start, s->grid,
-                           s->n, s->K, s->nbr, s->cnt, s->sc);
-    else
-        hipLaunchKernelGGL(k_neigh<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m.pos, s->hf, s->start, s->grid,
-                           s->n, s->K, s->nbr, s->cnt, s->sc);
-    if (e_mid) WCSPH_HIP(hipEventRecord(e_mid, s->st));
-    const int prec = s->cfg.precision;
-    if (prec == WCSPH_PREC_FP64) {
-        if (s->D == 3) launch_force<3, double>(s);
-        else launch_force<2, double>(s);
+    if (s->cfg.precision == WCSPH_PREC_FP64) {
+        neigh_lists(s);
+        if (e_mid) WCSPH_HIP(hipEventRecord(e_mid, s->st));
+        if (s->D == 3) launch_force64<3>(s);
+        else launch_force64<2>(s);
     } else {
-        if (s->D == 3) launch_force<3, float>(s);
-        else launch_force<2, float>(s);
+        if (e_mid) WCSPH_HIP(hipEventRecord(e_mid, s->st));
+        if (s->D == 3) nforce_dispatch<3>(s);
+        else nforce_dispatch<2>(s);
     }
     if (e_after) WCSPH_HIP(hipEventRecord(e_after, s->st));
     s->forced = true;
@@ -208,10 +272,11 @@ static void final_stage(wcsph_sim* s, int kick) {
     const Master& m = s->m[s->cur];
     if (s->D == 3)
         hipLaunchKernelGGL(k_final<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->dti, s->p.g,
-                           s->p.nu, s->p.rho_floor, kick, s->sc, s->n);
+                           s->p.nu, s->p.rho_floor, kick, s->sc, s->P, s->n);
     else
         hipLaunchKernelGGL(k_final<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->dti, s->p.g,
-                           s->p.nu, s->p.rho_floor, kick, s->sc, s->n);
+                           s->p.nu, s->p.rho_floor, kick, s->sc, s->P, s->n);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc, 2);
 }
 
 static hipEvent_t* step_events(wcsph_sim* s) {
@@ -325,6 +390,12 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         s->n = n;
         s->K = s->cfg.max_neighbours > 0 ? s->cfg.max_neighbours : 128;
         WCSPH_REQUIRE(s->K <= 4096, WCSPH_EINVAL, "max_neighbours above 4096");
+        s->R = s->cfg.cells_per_support == 0 ? 2 : s->cfg.cells_per_support;
+        WCSPH_REQUIRE(s->R == 1 || s->R == 2, WCSPH_EINVAL, "cells_per_support must be 1 or 2");
+        s->Q = s->cfg.queue_depth == 0 ? 16 : s->cfg.queue_depth;
+        WCSPH_REQUIRE(s->Q == 16 || s->Q == 32, WCSPH_EINVAL, "queue_depth must be 16 or 32");
+        // the fused pass packs a particle index and a row number in 32 bits
+        WCSPH_REQUIRE(n < ((int64_t)1 << 27), WCSPH_EINVAL, "too many particles for one simulation (limit 2^27)");
         s->device = p->device;
         WCSPH_HIP(hipSetDevice(s->device));
         WCSPH_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
@@ -344,8 +415,10 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         s->acc = s->alloc<double>(D * n);
         s->divv = s->alloc<double>(n);
         s->dti = s->alloc<double>(n);
-        s->hf = s->alloc<float>(n);
-        s->q4 = s->alloc<float4>(n);
+        s->pk.px = s->alloc<float4>(n);
+        s->pk.pv = s->alloc<float4>(n);
+        s->pk.pq = s->alloc<float4>(n);
+        s->pk.pcell = s->alloc<uint32_t>(n);
         if (s->cfg.precision == WCSPH_PREC_FP64) {
             s->pr64 = s->alloc<double>(n);
             s->c64 = s->alloc<double>(n);
@@ -353,9 +426,12 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         s->key = s->alloc<uint32_t>(n);
         s->slot = s->alloc<uint32_t>(n);
         s->perm = s->alloc<uint32_t>(n);
-        s->nbr = s->alloc<uint32_t>((int64_t)s->K * n);
-        s->cnt = s->alloc<uint32_t>(n);
         s->sc = s->alloc<Scalars>(1);
+        s->P.nblk = s->nb(n);
+        s->P.pairs = s->alloc<unsigned long long>(s->P.nblk);
+        s->P.maxcnt = s->alloc<uint32_t>(s->P.nblk);
+        s->P.over = s->alloc<uint32_t>(s->P.nblk);
+        s->P.dtmin = s->alloc<double>(s->P.nblk);
         s->diag_part = s->alloc<double>(s->nb(n) * kDiag);
         s->diag_out = s->alloc<double>(kDiag);
         WCSPH_HIP(hipMemsetAsync(s->acc, 0, sizeof(double) * D * n, s->st));
@@ -447,6 +523,17 @@ int wcsph_sim_last_force_ms(wcsph_sim* s, float* out) {
     });
 }
 
+int wcsph_sim_grid(wcsph_sim* s, double* lo, double* L, int32_t* nc) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && s->uploaded && lo && L && nc, WCSPH_EINVAL, "null argument or nothing uploaded");
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = s->grid.lo[d];
+            nc[d] = s->grid.nc[d];
+        }
+        *L = s->grid.L;
+    });
+}
+
 int wcsph_sim_set_timing(wcsph_sim* s, int32_t enable) {
     return guarded([&] {
         WCSPH_REQUIRE(s, WCSPH_EINVAL, "null simulation");
@@ -531,6 +618,12 @@ int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t 
     return guarded([&] {
         WCSPH_REQUIRE(s && s->forced && counts, WCSPH_ESTATE, "no force pass yet or null counts");
         WCSPH_REQUIRE(!lists || cap > 0, WCSPH_EINVAL, "cap must be positive with lists");
+        if (s->cfg.precision != WCSPH_PREC_FP64) {
+            // the fused pass keeps no lists: rebuild them for the current cell order
+            hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, s->st, s->sc, 2);
+            neigh_lists(s, 0);
+            WCSPH_HIP(hipGetLastError());
+        }
         WCSPH_HIP(hipStreamSynchronize(s->st));
         const int64_t n = s->n;
         const Master& m = s->m[s->cur];

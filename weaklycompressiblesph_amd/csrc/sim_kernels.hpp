@@ -5,9 +5,13 @@
 //   pos, vel, vh : FP64 [D][n]       master state (kick-drift in FP64)
 //   dens, mass, h: FP64 [n]
 //   type: u8 [n] (0 fluid, 1 wall), id: u32 [n] (index at upload)
-//   hf: FP32 [n] smoothing length, q4: float4 [n] {rho, P/rho^2, c, m}
+//   px: float4 [n] {x, y, z cell-local (x - cell origin), x cell index}
+//   pv: float4 [n] {vx, vy, vz, m}, pq: float4 [n] {rho, P/rho^2, c, h}
+//   pcell: u32 [n] cell key of each particle
 //   acc: FP64 [D][n], divv, dti: FP64 [n]   force outputs
-//   cell_start: u32 [ncell+1], nbr: u32 [K][n] (column per list slot), cnt: u32 [n]
+//   cell_start: u32 [ncell+1]
+//   FP64 path only: nbr u32 [K][n] (slot t of all i contiguous), cnt u32 [n],
+//   pr64/c64 FP64 [n]
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -21,7 +25,9 @@ constexpr int kBlock = 256;
 
 struct Grid {
     double lo[3];
-    double invL;
+    double invL;  // 1 / L
+    double L;     // cell edge (>= largest support / R)
+    float Lf;     // (float)L
     int nc[3];
 };
 
@@ -42,6 +48,16 @@ struct Scalars {  // device-resident step scalars
     long long step;
     unsigned long long pairs_last, pairs_total;
     unsigned int escaped, maxcnt, overflow, pad;
+};
+
+// Per-block partial results (one slot per block of the particle kernels),
+// reduced in index order by k_reduce_partials: no same-address atomics.
+struct Partials {
+    unsigned long long* pairs;  // pair count per block
+    uint32_t* maxcnt;           // largest per-particle count per block
+    uint32_t* over;             // list overflow flag per block
+    double* dtmin;              // time-step bound per block
+    int64_t nblk;
 };
 
 // Map blockIdx so consecutive logical blocks share an XCD (and its L2):
@@ -231,19 +247,51 @@ struct EosConsts {
 
 // Gather the master state into cell order and derive the force inputs
 // (CalcPressure, src/force.h:3-9 with Murnaghan, src/EoS.h:78-86).
+// Cell-local coordinate along one axis: x - (lo + c L), both steps in FP64
+// without contraction, rounded once to FP32. The oracle evaluates the same
+// expression (orc_neighbours_brute_cells).
+__device__ __forceinline__ float cell_local(double x, double lo, int c, double L) {
+    const double origin = __dadd_rn(lo, __dmul_rn((double)c, L));
+    return (float)__dsub_rn(x, origin);
+}
+
+template <int D>
+__device__ __forceinline__ void key_cell(const Grid& g, uint32_t key, int* c) {
+    c[0] = (int)(key % (uint32_t)g.nc[0]);
+    c[1] = (int)((key / (uint32_t)g.nc[0]) % (uint32_t)g.nc[1]);
+    c[2] = D == 3 ? (int)(key / ((uint32_t)g.nc[0] * (uint32_t)g.nc[1])) : 0;
+}
+
+struct Packed {  // FP32 force records in cell order
+    float4* px;
+    float4* pv;
+    float4* pq;
+    uint32_t* pcell;
+};
+
+// Gather the master state into cell order and derive the force inputs
+// (CalcPressure, src/force.h:3-9 with Murnaghan, src/EoS.h:78-86).
 // pr64/c64 (P/rho^2 and c in FP64) are written only for the FP64 path.
 template <int D>
-__global__ void k_reorder(const uint32_t* __restrict__ perm, Master src, Master dst, float* __restrict__ hf,
-                          float4* __restrict__ q4, double* __restrict__ pr64, double* __restrict__ c64,
+__global__ void k_reorder(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ key, Master src,
+                          Master dst, Packed pk, Grid g, double* __restrict__ pr64, double* __restrict__ c64,
                           EosConsts e, int64_t n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t o = perm[p];
+    const uint32_t k = key[o];
+    int c[3];
+    key_cell<D>(g, k, c);
+    float xl[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-        dst.pos[d * n + p] = src.pos[d * n + o];
-        dst.vel[d * n + p] = src.vel[d * n + o];
+        const double x = src.pos[d * n + o];
+        const double v = src.vel[d * n + o];
+        dst.pos[d * n + p] = x;
+        dst.vel[d * n + p] = v;
         dst.vh[d * n + p] = src.vh[d * n + o];
+        xl[d] = cell_local(x, g.lo[d], c[d], g.L);
+        vf[d] = (float)v;
     }
     const double rho = src.dens[o], m = src.mass[o], h = src.h[o];
     dst.dens[p] = rho;
@@ -256,17 +304,78 @@ __global__ void k_reorder(const uint32_t* __restrict__ perm, Master src, Master 
     P = P > 0.0 ? P : 0.0;
     double c2 = e.B * e.gamma / e.rho0 * pow(r, e.gamma - 1.0);
     c2 = c2 > 1.0e-16 ? c2 : 1.0e-16;
-    const double pr = P / (rho * rho), c = sqrt(c2);
-    hf[p] = (float)h;
-    q4[p] = make_float4((float)rho, (float)pr, (float)c, (float)m);
+    const double pr = P / (rho * rho), cs = sqrt(c2);
+    pk.px[p] = make_float4(xl[0], xl[1], xl[2], (float)c[0]);
+    pk.pv[p] = make_float4(vf[0], vf[1], vf[2], (float)m);
+    pk.pq[p] = make_float4((float)rho, (float)pr, (float)cs, (float)h);
+    pk.pcell[p] = k;
     if (pr64) {
         pr64[p] = pr;
-        c64[p] = c;
+        c64[p] = cs;
     }
 }
 
-// Wave-then-block reduction of (sum, max) into the step scalars.
-__device__ __forceinline__ void block_count_reduce(uint32_t cnt, uint32_t over, Scalars* s) {
+// Walks the (2R+1)^D cells around particle i's cell (R = cells per
+// support length), yielding the candidate index range of each non-empty
+// cell in turn, with the FP32 offset (c_i - c_j) L of that cell.
+// With R = 2 cells whose box is farther than hcut from i are skipped.
+template <int D, int R>
+struct Stencil {
+    static constexpr int S = 2 * R + 1;
+    static constexpr int NS = D == 3 ? S * S * S : S * S;
+    int k;
+    uint32_t p, pend;
+
+    __device__ __forceinline__ void advance(const Grid& g, const uint32_t* __restrict__ start, const int* c,
+                                            const float* xl, float hcut2) {
+        while (++k < NS) {
+            const int d0 = k % S - R, d1 = (k / S) % S - R, d2 = D == 3 ? k / (S * S) - R : 0;
+            const int c0 = c[0] + d0, c1 = c[1] + d1, c2 = c[2] + d2;
+            if (c0 < 0 || c0 >= g.nc[0] || c1 < 0 || c1 >= g.nc[1]) continue;
+            if (D == 3 && (c2 < 0 || c2 >= g.nc[2])) continue;
+            if (R > 1) {
+                const int dd[3] = {d0, d1, d2};
+                float gap2 = 0.f;
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const float lo = (float)dd[a] * g.Lf, hi = (float)(dd[a] + 1) * g.Lf;
+                    const float gp = fmaxf(fmaxf(lo - xl[a], xl[a] - hi), 0.f);
+                    gap2 += gp * gp;
+                }
+                if (gap2 > hcut2) continue;
+            }
+            const uint32_t cell = D == 3 ? (uint32_t)((c2 * g.nc[1] + c1) * g.nc[0] + c0)
+                                         : (uint32_t)(c1 * g.nc[0] + c0);
+            p = start[cell];
+            pend = start[cell + 1];
+            if (p < pend) return;
+        }
+    }
+    // FP32 offset (c_i - c_j) L for the current cell, per axis
+    __device__ __forceinline__ static float off(int kk, int axis, float Lf) {
+        const int d = axis == 0 ? kk % S - R : axis == 1 ? (kk / S) % S - R : kk / (S * S) - R;
+        return (float)(-d) * Lf;
+    }
+};
+
+// The neighbour predicate on cell-local coordinates (no contraction):
+// d = (xl_i - xl_j) + (c_i - c_j) L per axis, r2 = sum d*d, r2 < (h_i + h_j)^2.
+template <int D>
+__device__ __forceinline__ bool nbr_pred(const float* xi, float hi, const float4& pj, const float* off, float hj) {
+    const float pjv[3] = {pj.x, pj.y, pj.z};
+    float r2 = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const float t = __fsub_rn(xi[d], pjv[d]);
+        const float dd = __fadd_rn(t, off[d]);
+        r2 = __fadd_rn(r2, __fmul_rn(dd, dd));
+    }
+    const float H = __fadd_rn(hi, hj);
+    return r2 > 0.f && r2 < __fmul_rn(H, H);
+}
+
+// Wave-then-block reduction of (sum, max, or) into this block's partial slot.
+__device__ __forceinline__ void block_count_partial(uint32_t cnt, uint32_t over, const Partials& P, int64_t blk) {
     __shared__ unsigned long long sh_sum[kBlock / 64];
     __shared__ uint32_t sh_max[kBlock / 64];
     __shared__ uint32_t sh_over[kBlock / 64];
@@ -293,75 +402,245 @@ __device__ __forceinline__ void block_count_reduce(uint32_t cnt, uint32_t over, 
             m = sh_max[k] > m ? sh_max[k] : m;
             o |= sh_over[k];
         }
-        atomicAdd(&s->pairs_last, t);
-        atomicAdd(&s->pairs_total, t);
-        atomicMax(&s->maxcnt, m);
-        if (o) atomicOr(&s->overflow, 1u);
+        P.pairs[blk] = t;
+        P.maxcnt[blk] = m;
+        P.over[blk] = o;
     }
 }
 
-// Neighbour pass: for each i, every j != i in the 3^D cells around i's cell
-// with r2 > 0 and r2 < (h_i + h_j)^2, where the pair distance is formed as
-// (float)(x_i - x_j) per component and summed in FP32 without contraction.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_neigh(const double* __restrict__ pos, const float* __restrict__ hf,
-                                                  const uint32_t* __restrict__ start, Grid g, int64_t n,
-                                                  int K, uint32_t* __restrict__ nbr, uint32_t* __restrict__ cnt,
-                                                  Scalars* s) {
+// Reduce the per-block partials in index order into the step scalars.
+// what: bit 0 pair counts (added to pairs_last, and to pairs_total if
+// bit 2), bit 1 the dt bound (into dtmin_bits).
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(Partials P, Scalars* s, int what) {
+    __shared__ unsigned long long sh_p[kBlock];
+    __shared__ uint32_t sh_m[kBlock], sh_o[kBlock];
+    __shared__ double sh_d[kBlock];
+    unsigned long long p = 0;
+    uint32_t m = 0, o = 0;
+    double d = 1.0e30;
+    for (int64_t b = threadIdx.x; b < P.nblk; b += kBlock) {
+        if (what & 1) {
+            p += P.pairs[b];
+            m = P.maxcnt[b] > m ? P.maxcnt[b] : m;
+            o |= P.over[b];
+        }
+        if (what & 2) d = P.dtmin[b] < d ? P.dtmin[b] : d;
+    }
+    sh_p[threadIdx.x] = p;
+    sh_m[threadIdx.x] = m;
+    sh_o[threadIdx.x] = o;
+    sh_d[threadIdx.x] = d;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            sh_p[threadIdx.x] += sh_p[threadIdx.x + off];
+            sh_m[threadIdx.x] = sh_m[threadIdx.x + off] > sh_m[threadIdx.x] ? sh_m[threadIdx.x + off] : sh_m[threadIdx.x];
+            sh_o[threadIdx.x] |= sh_o[threadIdx.x + off];
+            sh_d[threadIdx.x] = sh_d[threadIdx.x + off] < sh_d[threadIdx.x] ? sh_d[threadIdx.x + off] : sh_d[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (what & 1) {
+            s->pairs_last += sh_p[0];
+            if (what & 4) s->pairs_total += sh_p[0];
+            s->maxcnt = sh_m[0] > s->maxcnt ? sh_m[0] : s->maxcnt;
+            s->overflow |= sh_o[0];
+        }
+        if (what & 2) s->dtmin_bits = (unsigned long long)__double_as_longlong(sh_d[0]);
+    }
+}
+
+// Neighbour lists (FP64 path and on-demand list download): every j with
+// the cell-local FP32 predicate, in stencil order, slot t of all i stored
+// contiguously.
+template <int D, int R>
+__global__ __launch_bounds__(kBlock) void k_neigh(Packed pk, const uint32_t* __restrict__ start, Grid g,
+                                                  int64_t n, int K, float hmax, uint32_t* __restrict__ nbr,
+                                                  uint32_t* __restrict__ cnt, Partials P) {
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     uint32_t c_out = 0;
     if (i < n) {
-        double xi[D];
+        const float4 xi4 = pk.px[i];
+        const float xi[3] = {xi4.x, xi4.y, xi4.z};
+        const float hi = pk.pq[i].w;
+        int c[3];
+        key_cell<D>(g, pk.pcell[i], c);
+        const float hc = (hi + hmax) * 1.0001f;
+        Stencil<D, R> st;
+        st.k = -1;
+        st.advance(g, start, c, xi, hc * hc);
+        while (st.k < Stencil<D, R>::NS) {
+            float off[3];
 #pragma unroll
-        for (int d = 0; d < D; ++d) xi[d] = pos[d * n + i];
-        const float hi = hf[i];
-        int c[3] = {0, 0, 0};
-        bool esc = false;
-        cell_of<D>(g, xi, c, esc);
-        const int zr = D == 3 ? 1 : 0;
-        const int xa = c[0] > 0 ? c[0] - 1 : 0;
-        const int xb = c[0] < g.nc[0] - 1 ? c[0] + 1 : g.nc[0] - 1;
-        for (int dz = -zr; dz <= zr; ++dz) {
-            const int zz = D == 3 ? c[2] + dz : 0;
-            if (D == 3 && (zz < 0 || zz >= g.nc[2])) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int yy = c[1] + dy;
-                if (yy < 0 || yy >= g.nc[1]) continue;
-                const int64_t row = D == 3 ? ((int64_t)zz * g.nc[1] + yy) * g.nc[0] : (int64_t)yy * g.nc[0];
-                const uint32_t jb = start[row + xa], je = start[row + xb + 1];
-                for (uint32_t j = jb; j < je; ++j) {
-                    if ((int64_t)j == i) continue;
-                    float r2 = 0.f;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) {
-                        const float dd = (float)(xi[d] - pos[d * n + j]);
-                        r2 = __fadd_rn(r2, __fmul_rn(dd, dd));
-                    }
-                    const float H = __fadd_rn(hi, hf[j]);
-                    if (r2 > 0.f && r2 < __fmul_rn(H, H)) {
-                        if ((int)c_out < K) nbr[(int64_t)c_out * n + i] = j;
-                        ++c_out;
-                    }
+            for (int d = 0; d < D; ++d) off[d] = Stencil<D, R>::off(st.k, d, g.Lf);
+            for (uint32_t j = st.p; j < st.pend; ++j) {
+                if (nbr_pred<D>(xi, hi, pk.px[j], off, pk.pq[j].w)) {
+                    if ((int)c_out < K) nbr[(int64_t)c_out * n + i] = j;
+                    ++c_out;
                 }
             }
+            st.advance(g, start, c, xi, hc * hc);
         }
         cnt[i] = c_out;
     }
-    block_count_reduce(c_out, c_out > (uint32_t)K ? 1u : 0u, s);
+    block_count_partial(c_out, c_out > (uint32_t)K ? 1u : 0u, P, blockIdx.x);
 }
 
-template <typename T>
-struct ForceArgs {
+// Fused neighbour + force pass (FP32). The wave walks the (2R+1)^(D-1)
+// rows of cells around its particles in lockstep (R = cells per support);
+// per row each lane clips its own x window to the chord of its support
+// sphere, so with R = 2 about 2.5 candidates are tested per neighbour.
+// Hits are queued in LDS (index | row << 27); whenever some lane's queue is
+// full the wave evaluates all queued pairs. No neighbour list reaches HBM.
+// The summation order per i is the scan order, so results are deterministic.
+struct NFArgs {
+    Packed pk;
+    const uint32_t* start;
+    Grid g;
+    int64_t n;
+    float hmax, nu;
+    double c_cfl;
+    PairConsts<float> k;
+    double* acc;
+    double* divv;
+    double* dti;
+    Partials P;
+};
+
+__device__ __forceinline__ float axis_gap(int d, float x, float Lf) {
+    const float lo = (float)d * Lf, hi = (float)(d + 1) * Lf;
+    return fmaxf(fmaxf(lo - x, x - hi), 0.f);
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+template <int D, int R, int Q, bool UH>
+__global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
+    constexpr int S = 2 * R + 1;
+    constexpr int NR = D == 3 ? S * S : S;  // rows of cells along x
+    constexpr int kShift = 27;
+    constexpr uint32_t kMask = (1u << kShift) - 1u;
+    __shared__ uint32_t qbuf[kBlock / 64][64 * (Q + 1)];
+    const int lane = threadIdx.x & 63;
+    uint32_t* q = &qbuf[threadIdx.x >> 6][lane * (Q + 1)];
+    const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    const bool live = i < a.n;
+    const float4 xi4 = live ? a.pk.px[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 vi4 = live ? a.pk.pv[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 qi4 = live ? a.pk.pq[i] : make_float4(1.f, 0.f, 1.f, 1.f);
+    const float xi[3] = {xi4.x, xi4.y, xi4.z};
+    const float vi[3] = {vi4.x, vi4.y, vi4.z};
+    const float cxi = xi4.w;
+    const IState<float> I{qi4.w, qi4.x, qi4.y, qi4.z, a.nu};
+    const float Lf = a.g.Lf, invLf = 1.0f / Lf;
+    int c[3] = {0, 0, 0};
+    if (live) key_cell<D>(a.g, a.pk.pcell[i], c);
+    const float hc = (UH ? 2.f * I.h : I.h + a.hmax) * 1.0001f;
+    const float hc2 = hc * hc;
+    float acc[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] = 0.f;
+    float divv = 0.f, vsig = 0.f;
+    uint32_t qn = 0, total = 0;
+
+    auto flush = [&]() {
+        const uint32_t qmax = wave_max(qn);
+        for (uint32_t t = 0; t < qmax; ++t) {
+            if (t < qn) {
+                const uint32_t e = q[t];
+                const uint32_t j = e & kMask;
+                const int r = (int)(e >> kShift);
+                const float4 pj = a.pk.px[j];
+                const float4 vj = a.pk.pv[j];
+                const float4 qj = a.pk.pq[j];
+                const float off[3] = {(cxi - pj.w) * Lf, (float)(R - r % S) * Lf,
+                                      D == 3 ? (float)(R - r / S) * Lf : 0.f};
+                const float pjv[3] = {pj.x, pj.y, pj.z};
+                const float vjv[3] = {vj.x, vj.y, vj.z};
+                float dr[D], dv[D];
+                float r2 = 0.f, drdv = 0.f;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    dr[d] = (xi[d] - pjv[d]) + off[d];
+                    dv[d] = vi[d] - vjv[d];
+                    r2 += dr[d] * dr[d];
+                    drdv += dr[d] * dv[d];
+                }
+                const JState<float> J{qj.w, qj.x, qj.y, qj.z, a.nu, vj.w};
+                float mu;
+                const float den = pair_mu(r2, drdv, I, J.h, a.k, mu, vsig);
+                pair_force<D>(dr, dv, r2, drdv, den, mu, I, J, a.k, acc, divv);
+            }
+        }
+        total += qn;
+        qn = 0;
+    };
+
+    for (int r = 0; r < NR; ++r) {
+        const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
+        uint32_t p0 = 0, len = 0;
+        const int cy = c[1] + dy, cz = c[2] + dz;
+        if (live && cy >= 0 && cy < a.g.nc[1] && (D == 2 || (cz >= 0 && cz < a.g.nc[2]))) {
+            const float gy = axis_gap(dy, xi[1], Lf);
+            const float gz = D == 3 ? axis_gap(dz, xi[2], Lf) : 0.f;
+            const float g2 = gy * gy + gz * gz;
+            if (g2 < hc2) {
+                const float w = sqrtf(hc2 - g2);
+                int xa = (int)floorf((xi[0] - w) * invLf), xb = (int)floorf((xi[0] + w) * invLf);
+                xa = max(max(xa, -R), -c[0]);
+                xb = min(min(xb, R), a.g.nc[0] - 1 - c[0]);
+                if (xa <= xb) {
+                    const uint32_t row = D == 3 ? (uint32_t)((cz * a.g.nc[1] + cy) * a.g.nc[0])
+                                                : (uint32_t)(cy * a.g.nc[0]);
+                    p0 = a.start[row + c[0] + xa];
+                    len = a.start[row + c[0] + xb + 1] - p0;
+                }
+            }
+        }
+        const float offy = (float)(-dy) * Lf, offz = (float)(-dz) * Lf;
+        const uint32_t tmax = wave_max(len);
+        for (uint32_t t = 0; t < tmax; ++t) {
+            if (t < len) {
+                const uint32_t j = p0 + t;
+                const float4 pj = a.pk.px[j];
+                const float hj = UH ? I.h : a.pk.pq[j].w;
+                const float off[3] = {(cxi - pj.w) * Lf, offy, offz};
+                if (nbr_pred<D>(xi, I.h, pj, off, hj)) {
+                    q[qn] = j | ((uint32_t)r << kShift);
+                    ++qn;
+                }
+            }
+            if (__any(qn == (uint32_t)Q)) flush();
+        }
+    }
+    flush();
+    if (live) {
+        const int64_t n = a.n;
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.acc[d * n + i] = (double)acc[d];
+        a.divv[i] = (double)divv / (double)I.rho;
+        a.dti[i] = a.c_cfl * (double)I.h / ((double)I.c + (double)vsig);
+    }
+    block_count_partial(total, 0u, a.P, blockIdx.x);
+}
+
+// FP64 force pass over neighbour lists (the GPU oracle path): pair
+// differences and all arithmetic in FP64 from the master arrays.
+struct ForceArgs64 {
     const double* pos;
     const double* vel;
-    const float* hf;
-    const float4* q4;
-    // FP64 path inputs (master arrays and FP64 EoS outputs)
-    const double* h64;
-    const double* rho64;
-    const double* m64;
-    const double* pr64;
-    const double* c64;
+    const double* h;
+    const double* rho;
+    const double* m;
+    const double* pr;
+    const double* c;
     const uint32_t* nbr;
     const uint32_t* cnt;
     double* acc;
@@ -371,12 +650,11 @@ struct ForceArgs {
     int K;
     double nu;
     double c_cfl;
-    PairConsts<T> k;
+    PairConsts<double> k;
 };
 
-// Force pass over the neighbour lists; T is the pair arithmetic type.
-template <int D, typename T>
-__global__ __launch_bounds__(kBlock) void k_force(ForceArgs<T> a) {
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_force64(ForceArgs64 a) {
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const int64_t n = a.n;
@@ -386,53 +664,42 @@ __global__ __launch_bounds__(kBlock) void k_force(ForceArgs<T> a) {
         xi[d] = a.pos[d * n + i];
         vi[d] = a.vel[d * n + i];
     }
-    IState<T> I;
-    if constexpr (sizeof(T) == 8) {
-        I = IState<T>{a.h64[i], a.rho64[i], a.pr64[i], a.c64[i], (T)a.nu};
-    } else {
-        const float4 qi = a.q4[i];
-        I = IState<T>{(T)a.hf[i], (T)qi.x, (T)qi.y, (T)qi.z, (T)a.nu};
-    }
-    T acc[D];
+    const IState<double> I{a.h[i], a.rho[i], a.pr[i], a.c[i], a.nu};
+    double acc[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) acc[d] = T(0);
-    T divv = T(0), vsig = T(0);
-    const int c = min((int)a.cnt[i], a.K);
-    for (int t = 0; t < c; ++t) {
+    for (int d = 0; d < D; ++d) acc[d] = 0.0;
+    double divv = 0.0, vsig = 0.0;
+    const int cnt = min((int)a.cnt[i], a.K);
+    for (int t = 0; t < cnt; ++t) {
         const uint32_t j = a.nbr[(int64_t)t * n + i];
-        T dr[D], dv[D];
-        T r2 = T(0), drdv = T(0);
+        double dr[D], dv[D];
+        double r2 = 0.0, drdv = 0.0;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            dr[d] = (T)(xi[d] - a.pos[d * n + j]);
-            dv[d] = (T)(vi[d] - a.vel[d * n + j]);
+            dr[d] = xi[d] - a.pos[d * n + j];
+            dv[d] = vi[d] - a.vel[d * n + j];
             r2 += dr[d] * dr[d];
             drdv += dr[d] * dv[d];
         }
-        JState<T> J;
-        if constexpr (sizeof(T) == 8) {
-            J = JState<T>{a.h64[j], a.rho64[j], a.pr64[j], a.c64[j], (T)a.nu, a.m64[j]};
-        } else {
-            const float4 qj = a.q4[j];
-            J = JState<T>{(T)a.hf[j], (T)qj.x, (T)qj.y, (T)qj.z, (T)a.nu, (T)qj.w};
-        }
-        T mu;
-        const T den = pair_mu(r2, drdv, I, J.h, a.k, mu, vsig);
+        const JState<double> J{a.h[j], a.rho[j], a.pr[j], a.c[j], a.nu, a.m[j]};
+        double mu;
+        const double den = pair_mu(r2, drdv, I, J.h, a.k, mu, vsig);
         pair_force<D>(dr, dv, r2, drdv, den, mu, I, J, a.k, acc, divv);
     }
 #pragma unroll
-    for (int d = 0; d < D; ++d) a.acc[d * n + i] = (double)acc[d];
-    a.divv[i] = (double)divv / (double)I.rho;                                   // src/force.h:57
-    a.dti[i] = a.c_cfl * (double)I.h / ((double)I.c + (double)vsig);           // src/force.h:56
+    for (int d = 0; d < D; ++d) a.acc[d * n + i] = acc[d];
+    a.divv[i] = divv / I.rho;                               // src/force.h:57
+    a.dti[i] = a.c_cfl * I.h / (I.c + vsig);               // src/force.h:56
 }
 
 // Gravity (src/init/Dam2.h:104-108), finalKick (src/class.h:160-163),
 // density floor (src/init/Dam2.h:111) and the per-particle time-step bound
-// of getTimeStepGlobal (src/integral.h:2-12) reduced with an atomic min.
+// of getTimeStepGlobal (src/integral.h:2-12), reduced to one value per block.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_final(Master m, double* __restrict__ acc,
                                                   const double* __restrict__ dti, double g, double nu,
-                                                  double rho_floor, int kick, Scalars* s, int64_t n) {
+                                                  double rho_floor, int kick, const Scalars* s, Partials P,
+                                                  int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double cand = 1.0e30;
     if (i < n) {
@@ -467,7 +734,7 @@ __global__ __launch_bounds__(kBlock) void k_final(Master m, double* __restrict__
     if (threadIdx.x == 0) {
         unsigned long long t = sh[0];
         for (int k = 1; k < (int)(blockDim.x >> 6); ++k) t = sh[k] < t ? sh[k] : t;
-        atomicMin(&s->dtmin_bits, t);
+        P.dtmin[blockIdx.x] = __longlong_as_double((long long)t);
     }
 }
 
