@@ -58,6 +58,65 @@ def cpu_baseline(sim, params, budget_s, threads):
                       f"on the developed {sim.n}-particle state, {t:.1f} s"}
 
 
+def accuracy_study(W, dim, dx, marks, chunk, early=100):
+    """Mixed and FP32 paths against the FP64 GPU path from the same initial
+    condition: dam front (max x of fluid within 2 dx of the floor), residual
+    column height (fluid within 2 dx of the left wall), run-up on the far
+    wall (fluid within 2 dx of it) at the time marks, energy drift, and
+    particle-wise errors after `early` steps (later, particle trajectories of
+    any two precisions decorrelate). Returns one JSON-able dict."""
+    import numpy as np
+    p = W.dam_break_params(dim)
+    desc = W.dam_break_desc(dim, dx)
+    st = W.dam_break_state(desc, p)
+    band, xwall = 2 * dx, desc.tank_l - 2 * dx
+    runs = {}
+    for prec in ("fp64", "fp32", "mixed"):
+        t_start = time.perf_counter()
+        sim = W.Simulation.from_state(p, st, precision=prec)
+        sim.init()
+        rows, todo, last = [], list(marks), time.time()
+        sim.step(early)  # particle-wise errors are compared here, before the flow
+        snap = sim.download()  # becomes sensitive to rounding
+        while todo:
+            sim.step(chunk)
+            s = sim.stats()
+            d = sim.diagnostics(band, band, xwall)
+            rows.append([s["time"], d["front"], d["column"], d["runup"], d["kinetic"] + d["potential"] + d["internal"],
+                         d["mass"], s["step"]])
+            if s["time"] >= todo[0]:
+                todo.pop(0)
+            if time.time() - last > 15:
+                print(f"# {prec}: t={s['time']:.3f} s, step {s['step']}", file=sys.stderr, flush=True)
+                last = time.time()
+        runs[prec] = {"rows": np.array(rows), "snap": snap, "wall_s": time.perf_counter() - t_start,
+                      "steps": int(rows[-1][6])}
+        sim.close()
+    ref = runs["fp64"]
+    out = {"workload": f"{dim}D dam break dx={dx}, {len(st['dens'])} particles", "marks_s": list(marks), "paths": {}}
+    for prec, r in runs.items():
+        R = r["rows"]
+        e0 = R[0, 4]
+        entry = {"steps": r["steps"], "wall_s": r["wall_s"], "ms_per_step": 1e3 * r["wall_s"] / r["steps"],
+                 "energy_drift": float((R[-1, 4] - e0) / abs(e0)), "mass_drift": float((R[-1, 5] - R[0, 5]) / R[0, 5])}
+        for k, name in ((1, "front"), (2, "column"), (3, "runup")):
+            vals = np.interp(marks, R[:, 0], R[:, k])
+            entry[name] = [float(v) for v in vals]
+            if prec != "fp64":
+                rv = np.interp(marks, ref["rows"][:, 0], ref["rows"][:, k])
+                entry[name + "_rel_err"] = [float(abs(a - b) / abs(b)) for a, b in zip(vals, rv)]
+        if prec != "fp64":
+            a, b = r["snap"], ref["snap"]
+            disp = b["pos"] - st["pos"]
+            entry[f"after_{early}_steps"] = {
+                "pos_rel_err": float(np.linalg.norm(a["pos"] - b["pos"]) / np.linalg.norm(disp)),
+                "vel_rel_err": float(np.linalg.norm(a["vel"] - b["vel"]) / np.linalg.norm(b["vel"])),
+                "dens_rel_err": float(np.linalg.norm(a["dens"] - b["dens"]) / np.linalg.norm(b["dens"] - st["dens"])),
+                "acc_rel_err": float(np.linalg.norm(a["acc"] - b["acc"]) / np.linalg.norm(b["acc"]))}
+        out["paths"][prec] = entry
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +128,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cells-per-support", type=int, default=0, help="1 or 2 (0: library default)")
-    ap.add_argument("--queue-depth", type=int, default=0, help="16 or 32 (0: library default)")
+    ap.add_argument("--queue-depth", type=int, default=0, help="24 or 32 (0: library default)")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="run the FP32 and mixed paths against the FP64 path to --t-end and print their errors")
+    ap.add_argument("--t-end", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -80,6 +142,10 @@ def main():
 
     dim = args.dim
     dx = args.dx or (0.0126 if dim == 3 else 0.02)
+    if args.accuracy:
+        marks = [m for m in (0.25, 0.5, 1.0) if m <= args.t_end] or [args.t_end]
+        print(json.dumps(accuracy_study(W, dim, dx, marks, 50)), flush=True)
+        return
     params = W.dam_break_params(dim)
     desc = W.dam_break_desc(dim, dx)
     st = W.dam_break_state(desc, params)
@@ -128,7 +194,7 @@ def main():
         "config": {"workload": f"{dim}D dam break, dx={dx} m, {n} particles ({st['n_fluid']} fluid), "
                                f"{args.precision} force path, full device-resident steps",
                    "particles": n, "dx": dx, "precision": args.precision,
-                   "cells_per_support": args.cells_per_support or 2, "queue_depth": args.queue_depth or 16,
+                   "cells_per_support": args.cells_per_support or 2, "queue_depth": args.queue_depth or 24,
                    "step_window":
                        f"steps {args.warmup + 1}-{args.warmup + K}"},
         "steps_per_s": K / wall,

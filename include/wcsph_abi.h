@@ -113,7 +113,7 @@ typedef struct {
     double box_lo[3];       /* cell-grid domain; particles outside are clamped */
     double box_hi[3];       /*   into the edge cells and counted as escaped */
     int32_t cells_per_support; /* cell edge = largest support / this (1 or 2; 0: 2) */
-    int32_t queue_depth;    /* fused pass: LDS queue slots per lane (16 or 32; 0: 16) */
+    int32_t queue_depth;    /* fused pass: LDS queue slots per lane (24 or 32; 0: 24) */
 } wcsph_sim_config;
 
 typedef struct {
@@ -157,6 +157,11 @@ int wcsph_sim_init(wcsph_sim* s);
 /* nsteps steps of src/main.cpp:89-133 (no Shepard filter), queued on the
  * simulation's stream; returns without waiting. */
 int wcsph_sim_step(wcsph_sim* s, int32_t nsteps);
+/* Steps are replayed from a captured two-step hipGraph (one submission per
+ * two steps) unless per-step timing is on or graphs are disabled here. */
+int wcsph_sim_set_graphs(wcsph_sim* s, int32_t enable);
+/* Non-blocking: *idle = 1 when all queued work of the simulation is done. */
+int wcsph_sim_query(wcsph_sim* s, int32_t* idle);
 /* Re-bin and run one force pass on the current state (no integration):
  * acc (hydro only, no gravity), div_v and dt_i of the current positions. */
 int wcsph_sim_force(wcsph_sim* s);
@@ -171,10 +176,12 @@ int wcsph_sim_download(wcsph_sim* s, double* pos, double* vel, double* dens, dou
  * not NULL) of the last force pass, in original order; lists is [n][cap]. */
 int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t cap);
 /* FP64 reductions on the device: out[0] mass, out[1..3] momentum,
- * out[4] kinetic, out[5] potential, out[6] Tait internal energy,
+ * out[4] kinetic, out[5] potential, out[6] internal energy of the clamped
+ * Tait law (zero below rho0),
  * out[7] dam front (max x of fluid with vertical coordinate < zfront),
- * out[8] column height (max vertical coordinate of fluid with x < xcol). */
-int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double* out);
+ * out[8] column height (max vertical coordinate of fluid with x < xcol),
+ * out[9] run-up (max vertical coordinate of fluid with x > xwall). */
+int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double xwall, double* out);
 /* The cell grid (for checking the neighbour predicate, which is defined
  * on cell-local coordinates): lo[3], cell edge *L, cells per axis nc[3]. */
 int wcsph_sim_grid(wcsph_sim* s, double* lo, double* L, int32_t* nc);
@@ -189,6 +196,41 @@ int wcsph_sim_last_force_ms(wcsph_sim* s, float* out);
  * out[4] whole steps -- with *nsteps timed, and restarts the sums. */
 int wcsph_sim_set_timing(wcsph_sim* s, int32_t enable);
 int wcsph_sim_timing(wcsph_sim* s, double* out, int64_t* nsteps);
+
+/* ---------------- 3. serving loop (continuous batching) ---------------- */
+/* One dam-break request: its physics, geometry (the particle count follows
+ * from geom), precision, number of steps, and arrival time in seconds after
+ * the start of wcsph_serve. */
+typedef struct {
+    wcsph_params params;
+    wcsph_dam_break_desc geom;
+    int32_t precision;
+    int32_t steps;
+    double arrival_s;
+} wcsph_request;
+
+typedef struct {
+    double arrival_s, start_s, end_s;   /* wall clock, seconds after the start */
+    int64_t n, steps, pairs;            /* particles, steps run, pair interactions */
+    double sim_time, front, column, kinetic;  /* state at completion */
+    int32_t status;                     /* 0, or the WCSPH_E* code that ended it */
+    int32_t precision_used;             /* WCSPH_PREC_*: a mixed request whose grid
+                                           exceeds the mixed path's 2048 cells
+                                           along x runs in FP32 */
+} wcsph_request_result;
+
+typedef struct {
+    int64_t hbm_budget_bytes;  /* admission limit on the summed device footprint */
+    int32_t max_slots;         /* requests running at once */
+    int32_t chunk_steps;       /* steps queued per request per scheduling pass */
+} wcsph_serve_config;
+
+/* Run the whole request stream: admit arrived requests (FIFO) while a slot
+ * is free and the budget allows, step each admitted request on its own
+ * stream from a captured two-step graph, retire finished ones and record
+ * per-request timing (latency = end_s - arrival_s) in out[] (same order as
+ * reqs). Returns when every request is done. */
+int wcsph_serve(const wcsph_serve_config* cfg, const wcsph_request* reqs, int32_t n, wcsph_request_result* out);
 
 #ifdef __cplusplus
 }

@@ -229,14 +229,12 @@ int orc_neighbours_brute_cells(int dim, int64_t n, const double* pos, const floa
         int c = 0;
         for (int64_t j = 0; j < n; ++j) {
             if (j == i) continue;
-            float r2 = 0.0f;
-            for (int k = 0; k < dim; ++k) {
-                const float t = xl[3 * i + k] - xl[3 * j + k];
-                const float off = (float)(ci[3 * i + k] - ci[3 * j + k]) * Lf;
-                const float d = t + off;
-                const float d2 = d * d;
-                r2 = r2 + d2;
-            }
+            float d[3] = {0.0f, 0.0f, 0.0f};
+            for (int k = 0; k < dim; ++k)
+                d[k] = fmaf((float)(ci[3 * i + k] - ci[3 * j + k]), Lf, xl[3 * i + k]) - xl[3 * j + k];
+            float r2 = d[0] * d[0];
+            r2 = fmaf(d[1], d[1], r2);
+            if (dim == 3) r2 = fmaf(d[2], d[2], r2);
             const float H = h[i] + h[j];
             const float H2 = H * H;
             if (r2 > 0.0f && r2 < H2) {
@@ -596,9 +594,11 @@ void orc_conservation(const orc_sim_params* p, const orc_state* s, double* out) 
         }
         out[4] += 0.5 * m * v2;
         out[5] += m * p->g * s->x[(D - 1) * n + i];
-        /* e(rho) = B/rho0 [ (rho/rho0)^(g-1)/(g-1) + rho0/rho - g/(g-1) ], de/drho = P/rho^2 */
+        /* internal energy of the clamped Tait law: de/drho = P/rho^2, e(rho0) = 0,
+         * e = B/rho0 [ r^(g-1)/(g-1) + 1/r - g/(g-1) ] for r = rho/rho0 >= 1, 0 below */
         const double r = s->dens[i] / p->rho0;
-        out[6] += m * B / p->rho0 * (pow(r, p->gamma - 1.0) / (p->gamma - 1.0) + 1.0 / r - p->gamma / (p->gamma - 1.0));
+        if (r > 1.0)
+            out[6] += m * B / p->rho0 * (pow(r, p->gamma - 1.0) / (p->gamma - 1.0) + 1.0 / r - p->gamma / (p->gamma - 1.0));
     }
 }
 

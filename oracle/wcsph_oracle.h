@@ -72,10 +72,11 @@ int orc_neighbours_brute(int dim, int64_t n, const double* pos, const float* h,
 /* The device-resident path's neighbour predicate, defined on a cell grid
  * (origin lo, cell edge L, nc cells per axis; cells of size L >= support):
  * c = clamp(floor((x - lo) / L)), origin = lo + c L (FP64, no contraction),
- * xl = (float)(x - origin); for a pair, d = (xl_i - xl_j) + (float)(c_i - c_j)
- * * (float)L per component (FP32), r2 = sum of d*d (FP32, no contraction);
- * neighbour iff r2 > 0 && r2 < (h_i + h_j)^2 (FP32). Same outputs as
- * orc_neighbours_brute. */
+ * xl = (float)(x - origin); for a pair, per axis
+ * d = fmaf((float)(c_i - c_j), (float)L, xl_i) - xl_j, then
+ * r2 = fmaf(d_z, d_z, fmaf(d_y, d_y, d_x * d_x)) (all FP32, fused where
+ * written, nothing else contracted); neighbour iff r2 > 0 && r2 < H * H with
+ * H = h_i + h_j (FP32). Same outputs as orc_neighbours_brute. */
 int orc_neighbours_brute_cells(int dim, int64_t n, const double* pos, const float* h,
                                const double* lo, double L, const int* nc,
                                int* counts, int* lists, int cap);

@@ -1,0 +1,94 @@
+"""GPU checks of the step graphs and of the serving loop.
+
+* graph replay: steps replayed from the captured two-step hipGraph equal
+  eagerly launched steps bit for bit, replay after replay, with downloads
+  and synchronisations in between;
+* serving: every request of a mixed stream gives bit-identical results to
+  running it alone (no shared device state, deterministic kernels); FIFO
+  admission respects the slot limit and the HBM budget.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_graph_replay_matches_eager(W, precision):
+    p = W.dam_break_params(3)
+    st = W.dam_break_state(W.dam_break_desc(3, 0.04, tank_l=2.0, col_h=1.0, tank_h=1.5), p)
+    outs = {}
+    for graphs in (True, False):
+        with W.Simulation.from_state(p, st, precision=precision) as sim:
+            sim.set_graphs(graphs)
+            sim.init()
+            snaps = []
+            for _ in range(4):
+                sim.step(10)
+                snaps.append(sim.download())
+                sim.step(3)  # odd counts end with an eager step
+                sim.sync()
+            outs[graphs] = (snaps, sim.stats())
+    for a, b in zip(outs[True][0], outs[False][0]):
+        for k in a:
+            assert np.array_equal(a[k], b[k]), k
+    assert outs[True][1]["pairs_total"] == outs[False][1]["pairs_total"]
+
+
+def small_stream(W, precision="mixed"):
+    spec = [(2, 30_000, 2.0, 0.05, 160, 0.00), (3, 25_000, 1.5, 0.12, 120, 0.02),
+            (2, 60_000, 3.5, 0.18, 200, 0.05), (3, 40_000, 2.5, 0.08, 90, 0.06),
+            (2, 20_000, 1.0, 0.02, 150, 0.30)]
+    return [W.make_request(d, n, a, al, s, t, precision) for d, n, a, al, s, t in spec]
+
+
+KEYS = ("steps", "pairs", "sim_time", "front", "column", "kinetic", "n")
+
+
+def test_serving_results_equal_alone(W):
+    reqs = small_stream(W)
+    mix = W.serve(reqs, max_slots=3, chunk_steps=25)
+    assert all(r["status"] == 0 for r in mix)
+    for k, r in enumerate(reqs):
+        r.arrival_s = 0.0
+        alone = W.serve([r], max_slots=1, chunk_steps=25)[0]
+        for key in KEYS:
+            assert alone[key] == mix[k][key], (k, key)
+        assert mix[k]["steps"] == r.steps
+        assert mix[k]["latency_s"] > 0 and mix[k]["start_s"] >= mix[k]["arrival_s"] - 1e-9
+
+
+def test_serving_fifo_one_slot(W):
+    reqs = small_stream(W, precision="fp32")
+    res = W.serve(reqs, max_slots=1, chunk_steps=50)
+    order = np.argsort([r["arrival_s"] for r in res], kind="stable")
+    for a, b in zip(order[:-1], order[1:]):
+        assert res[b]["start_s"] >= res[a]["end_s"] - 1e-9
+
+
+def test_serving_budget_serialises(W):
+    import ctypes as C
+    reqs = small_stream(W, precision="fp32")
+    # a budget that holds any one request but never the two largest at once
+    sizes = []
+    for r in reqs:
+        nf, nw = C.c_int64(), C.c_int64()
+        W.check(W._abi.lib().wcsph_dam_break_count(C.byref(r.geom), C.byref(nf), C.byref(nw)))
+        cfg = W.SimConfig()
+        cfg.precision = r.precision
+        sizes.append(W._abi.lib().wcsph_sim_bytes(r.geom.dim, nf.value + nw.value, C.byref(cfg)))
+    budget = max(sizes) + min(sizes) - 1
+    res = W.serve(reqs, max_slots=4, hbm_budget_bytes=budget, chunk_steps=50)
+    ev = sorted([(r["start_s"], 1, sizes[k]) for k, r in enumerate(res)] + [(r["end_s"], -1, sizes[k]) for k, r in
+                enumerate(res)], key=lambda e: (e[0], e[1]))
+    used = 0
+    for _, sign, sz in ev:
+        used += sign * sz
+        assert used <= budget
+    assert all(r["status"] == 0 for r in res)
+
+
+def test_serving_rejects_oversized_request(W):
+    reqs = small_stream(W)[:1]
+    with pytest.raises(W.WcsphError, match="budget"):
+        W.serve(reqs, hbm_budget_bytes=1 << 20)

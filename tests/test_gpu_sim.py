@@ -59,7 +59,7 @@ def test_neighbour_sets_bit_exact(O, W, dim, dx, geom, cps):
     assert st["pairs_last"] == int(rc.sum()) and st["overflow"] == 0 and st["escaped"] == 0
 
 
-@pytest.mark.parametrize("precision,tol,cps,queue", [("fp32", 1e-5, 2, 16), ("fp32", 1e-5, 1, 32),
+@pytest.mark.parametrize("precision,tol,cps,queue", [("fp32", 1e-5, 2, 24), ("fp32", 1e-5, 1, 32),
                                                      ("fp32", 1e-5, 2, 32), ("fp64", 1e-12, 2, 0)])
 @pytest.mark.parametrize("dim,dx,geom", [(2, 0.02, {}), (3, 0.04, SMALL3)])
 def test_force_pass_matches_oracle(O, W, dim, dx, geom, precision, tol, cps, queue):
@@ -170,7 +170,7 @@ def test_deterministic_and_walls_fixed(O, W):
 def test_diagnostics_match_oracle(O, W):
     s, p, op = developed(O, W, 2, 0.02, 20)
     with W.Simulation.from_state(p, state_dict(s)) as sim:
-        d = sim.diagnostics(zfront=0.04, xcol=0.04)
+        d = sim.diagnostics(zfront=0.04, xcol=0.04, xwall=0.5)
     ref = s.conservation(op)
     got = [d[k] for k in ("mass", "mom_x", "mom_y", "kinetic", "potential", "internal")]
     exp = [ref[0], ref[1], ref[2], ref[4], ref[5], ref[6]]
@@ -178,3 +178,5 @@ def test_diagnostics_match_oracle(O, W):
     assert d["front"] == s.dam_front(0.04)
     fl = (s.type == 0) & (s.x[0] < 0.04)
     assert d["column"] == s.x[1][fl].max()
+    fr = (s.type == 0) & (s.x[0] > 0.5)
+    assert d["runup"] == s.x[1][fr].max()

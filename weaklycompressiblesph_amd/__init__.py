@@ -206,11 +206,22 @@ class Simulation:
         check(lib().wcsph_sim_neighbours(self._h, ptr(counts), ptr(lists), cap))
         return (counts, lists) if cap else counts
 
-    def diagnostics(self, zfront, xcol):
-        out = np.zeros(9)
-        check(lib().wcsph_sim_diagnostics(self._h, zfront, xcol, ptr(out)))
-        keys = ["mass", "mom_x", "mom_y", "mom_z", "kinetic", "potential", "internal", "front", "column"]
+    def diagnostics(self, zfront, xcol, xwall=1.0e300):
+        """FP64 device reductions: conserved totals, dam front (max x of fluid
+        below zfront), column height (max height of fluid at x < xcol) and
+        run-up (max height of fluid at x > xwall)."""
+        out = np.zeros(10)
+        check(lib().wcsph_sim_diagnostics(self._h, zfront, xcol, xwall, ptr(out)))
+        keys = ["mass", "mom_x", "mom_y", "mom_z", "kinetic", "potential", "internal", "front", "column", "runup"]
         return dict(zip(keys, out))
+
+    def set_graphs(self, enable=True):
+        check(lib().wcsph_sim_set_graphs(self._h, 1 if enable else 0))
+
+    def idle(self):
+        v = C.c_int32()
+        check(lib().wcsph_sim_query(self._h, C.byref(v)))
+        return bool(v.value)
 
     def grid(self):
         lo = np.zeros(3)
@@ -236,3 +247,73 @@ class Simulation:
         out = np.zeros(3, np.float32)
         check(lib().wcsph_sim_last_force_ms(self._h, ptr(out)))
         return {"binning": float(out[0]), "neighbour": float(out[1]), "force": float(out[2])}
+
+
+# ----------------------------------------------------------------------------
+# serving loop
+# ----------------------------------------------------------------------------
+from ._abi import Request, RequestResult, ServeConfig  # noqa: E402
+
+
+def request_geometry(dim, n_target, aspect, *, col_w=0.5, depth=0.25):
+    """Dam-break geometry of a serving request: column col_w wide and
+    aspect*col_w high (depth deep in 3D) in a tank 3 col_w long and
+    1.5 aspect*col_w high, with dx chosen so the lattice holds about
+    n_target particles (fluid and wall)."""
+    col_h = aspect * col_w
+    base = dict(col_w=col_w, col_h=col_h, depth=depth, tank_l=3 * col_w, tank_h=1.5 * col_h)
+    vol = col_w * col_h * (depth if dim == 3 else 1.0)
+    dx = (vol / (0.5 * n_target)) ** (1.0 / dim)
+    for _ in range(6):  # secant-style refinement on the total count
+        nf, nw = C.c_int64(), C.c_int64()
+        check(lib().wcsph_dam_break_count(C.byref(dam_break_desc(dim, dx, **base)), C.byref(nf), C.byref(nw)))
+        n = nf.value + nw.value
+        if abs(n - n_target) < 0.02 * n_target:
+            break
+        dx *= (n / n_target) ** (1.0 / dim)
+    return dam_break_desc(dim, dx, **base)
+
+
+def make_request(dim, n_target, aspect, alpha, steps, arrival_s, precision="mixed", device=0):
+    geom = request_geometry(dim, n_target, aspect)
+    p = dam_break_params(dim, col_h=geom.col_h, alpha=alpha, device=device)
+    r = Request()
+    r.params, r.geom = p, geom
+    r.precision = PRECISIONS[precision]
+    r.steps, r.arrival_s = int(steps), float(arrival_s)
+    return r
+
+
+def poisson_workload(seed, count, rate, *, n_range=(20_000, 2_000_000), aspect=(1.0, 4.0), alpha=(0.02, 0.2),
+                     steps=(500, 5000), dims=(2, 3), precision="mixed"):
+    """Seeded request stream: Poisson arrivals at `rate` per second, dimension
+    uniform over dims, particle count log-uniform over n_range, column
+    aspect, alpha and step count uniform over their ranges."""
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    for _ in range(count):
+        t += rng.exponential(1.0 / rate)
+        dim = int(rng.choice(dims))
+        n = float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        reqs.append(make_request(dim, n, rng.uniform(*aspect), rng.uniform(*alpha), int(rng.integers(steps[0],
+                                 steps[1] + 1)), t, precision))
+    return reqs
+
+
+def serve(requests, *, hbm_budget_bytes=64 << 30, max_slots=4, chunk_steps=50):
+    """Run a request stream through the continuous-batching loop. Returns one
+    dict per request (same order): arrival/start/end wall times, latency,
+    particles, steps, pair interactions and the final diagnostics."""
+    n = len(requests)
+    arr = (Request * max(n, 1))(*requests)
+    out = (RequestResult * max(n, 1))()
+    cfg = ServeConfig(int(hbm_budget_bytes), int(max_slots), int(chunk_steps))
+    check(lib().wcsph_serve(C.byref(cfg), C.cast(arr, C.c_void_p), n, C.cast(out, C.c_void_p)))
+    res = []
+    for k in range(n):
+        o = out[k]
+        d = {f: getattr(o, f) for f, _ in RequestResult._fields_}
+        d["latency_s"] = o.end_s - o.arrival_s
+        d["dim"] = requests[k].geom.dim
+        res.append(d)
+    return res

@@ -50,6 +50,25 @@ class DamBreakDesc(C.Structure):
                 ("hfac", C.c_double)]
 
 
+class Request(C.Structure):
+    """wcsph_request."""
+    _fields_ = [("params", Params), ("geom", DamBreakDesc), ("precision", C.c_int32), ("steps", C.c_int32),
+                ("arrival_s", C.c_double)]
+
+
+class RequestResult(C.Structure):
+    """wcsph_request_result."""
+    _fields_ = [("arrival_s", C.c_double), ("start_s", C.c_double), ("end_s", C.c_double), ("n", C.c_int64),
+                ("steps", C.c_int64), ("pairs", C.c_int64), ("sim_time", C.c_double), ("front", C.c_double),
+                ("column", C.c_double), ("kinetic", C.c_double), ("status", C.c_int32),
+                ("precision_used", C.c_int32)]
+
+
+class ServeConfig(C.Structure):
+    """wcsph_serve_config."""
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_slots", C.c_int32), ("chunk_steps", C.c_int32)]
+
+
 def epi_dtype(dim):
     """EPI::Hydro (src/class.h:167-176)."""
     return np.dtype([("pos", "<f8", (dim,)), ("vel", "<f8", (dim,)), ("smth", "<f8"), ("dens", "<f8"),
@@ -76,7 +95,7 @@ EXPORTS = [
     "wcsph_sim_create", "wcsph_sim_destroy", "wcsph_sim_upload", "wcsph_sim_init", "wcsph_sim_step",
     "wcsph_sim_force", "wcsph_sim_sync", "wcsph_sim_get_stats", "wcsph_sim_download", "wcsph_sim_neighbours",
     "wcsph_sim_diagnostics", "wcsph_sim_last_force_ms", "wcsph_sim_set_timing", "wcsph_sim_timing",
-    "wcsph_sim_grid",
+    "wcsph_sim_grid", "wcsph_sim_set_graphs", "wcsph_sim_query", "wcsph_serve",
 ]
 
 _lib = None
@@ -118,10 +137,13 @@ def lib():
         "wcsph_sim_get_stats": (C.c_int, [VP, C.POINTER(SimStats)]),
         "wcsph_sim_download": (C.c_int, [VP, VP, VP, VP, VP, VP, VP]),
         "wcsph_sim_neighbours": (C.c_int, [VP, VP, VP, I32]),
-        "wcsph_sim_diagnostics": (C.c_int, [VP, F64, F64, VP]),
+        "wcsph_sim_diagnostics": (C.c_int, [VP, F64, F64, F64, VP]),
         "wcsph_sim_last_force_ms": (C.c_int, [VP, VP]),
         "wcsph_sim_set_timing": (C.c_int, [VP, I32]),
         "wcsph_sim_grid": (C.c_int, [VP, VP, VP, VP]),
+        "wcsph_sim_set_graphs": (C.c_int, [VP, I32]),
+        "wcsph_sim_query": (C.c_int, [VP, C.POINTER(I32)]),
+        "wcsph_serve": (C.c_int, [C.POINTER(ServeConfig), VP, I32, VP]),
         "wcsph_sim_timing": (C.c_int, [VP, VP, C.POINTER(I64)]),
     }
     for name, (res, args) in sig.items():

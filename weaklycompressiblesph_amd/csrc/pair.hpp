@@ -80,4 +80,41 @@ __device__ __forceinline__ void pair_force(const T* dr, const T* dv, T r2, T drd
     divv -= mF * drdv;
 }
 
+// Uniform-h form of the same pair body (both particles have smoothing
+// length h): H = 2h and 1/H^(D+1) are constants (hc.invH, hc.invHD1), the
+// regulariser is eps h^2 (hc.eps_h2), and one reciprocal of
+// den (rho_i + rho_j) serves both 1/den and 1/(rho_i + rho_j). VS selects the
+// signal-velocity rule at compile time (0: max(v_sig, mu), which stays 0;
+// 1: max(v_sig, -mu)).
+struct UniformH {
+    float h, eps_h2, invH, invHD1;
+};
+
+template <int D, int VS>
+__device__ __forceinline__ void pair_uniform(const float* dr, const float* dv, float r2, float drdv,
+                                             const IState<float>& I, const JState<float>& J,
+                                             const PairConsts<float>& k, const UniformH& hc, float* acc,
+                                             float& divv, float& vsig) {
+    const float den = r2 + hc.eps_h2;
+    const float rho_s = I.rho + J.rho;
+    const float X = fast_rcp(den * rho_s);              // 1 / (den (rho_i + rho_j))
+    const float mneg = fminf(drdv, 0.f);
+    const float mu = hc.h * mneg * (X * rho_s);          // (h_i + h_j)/2 = h
+    if (VS == 1) vsig = fmaxf(vsig, -mu);
+    const float cbar = 0.5f * (I.c + J.c);
+    const float Pi = (-k.alpha * cbar * mu + k.beta * mu * mu) * (2.f * X * den);  // / rhobar
+    const float invr = fast_rsqrt(r2);
+    const float s = r2 * invr * hc.invH;
+    const float a = fmaxf(1.f - s, 0.f);
+    const float b = fmaxf(0.5f - s, 0.f);
+    const float g = -3.f * a * a + 12.f * b * b;
+    const float mF = J.m * k.cnorm * hc.invHD1 * g * invr;  // m_j F
+    const float pt = I.pr + J.pr + Pi;
+    const float vc = mF * 2.f * (I.nu + J.nu) * r2 * X;
+    const float cp = -mF * pt;
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] += cp * dr[d] + vc * dv[d];
+    divv -= mF * drdv;
+}
+
 }  // namespace wcsph

@@ -16,6 +16,7 @@
 #include <cstring>
 
 #include "common.hpp"
+#include "mixed_kernels.hpp"
 #include "sim_kernels.hpp"
 
 using namespace wcsph;
@@ -40,6 +41,8 @@ struct wcsph_sim {
     float hmax = 0.f;
     bool uniform_h = false;  // all smoothing lengths equal (fixed in time)
     double *pr64 = nullptr, *c64 = nullptr;  // FP64 path only
+    Mixed mx{};                               // mixed path only
+    MixedConsts mc{};
     uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr,
              *part = nullptr;
     uint32_t *nbr = nullptr, *cnt = nullptr;
@@ -49,6 +52,8 @@ struct wcsph_sim {
     double* diag_out = nullptr;
     bool uploaded = false, forced = false;
     bool timing = false;
+    bool use_graphs = true;
+    hipGraphExec_t graph2[2] = {nullptr, nullptr};  // two steps starting from cur = 0 / 1
     std::vector<hipEvent_t> tev;  // 5 events per timed step, pooled
     int64_t tsteps = 0;
     std::vector<void*> allocs;
@@ -66,6 +71,8 @@ struct wcsph_sim {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : tev) (void)hipEventDestroy(e);
+        for (auto& g : graph2)
+            if (g) (void)hipGraphExecDestroy(g);
         if (st) (void)hipStreamDestroy(st);
     }
     int64_t nb(int64_t items) const { return ceil_div(items, kBlock); }
@@ -154,12 +161,23 @@ static void bin_and_reorder(wcsph_sim* s) {
     hipLaunchKernelGGL(k_cell_sort, dim3(s->nb(s->ncell)), dim3(kBlock), 0, s->st, s->start, s->perm, s->ncell);
     const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
     EosConsts e{s->p.rho0, B, s->p.gamma};
-    if (s->D == 3)
-        hipLaunchKernelGGL(k_reorder<3>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key, src, dst, s->pk,
-                           s->grid, s->pr64, s->c64, e, n);
-    else
-        hipLaunchKernelGGL(k_reorder<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key, src, dst, s->pk,
-                           s->grid, s->pr64, s->c64, e, n);
+    const bool mixed = s->cfg.precision == WCSPH_PREC_MIXED;
+    const float vscale = mixed ? (float)(1.0 / s->p.c0) : 1.0f;
+    if (s->D == 3) {
+        if (mixed)
+            hipLaunchKernelGGL((k_reorder<3, MixedExtra<3>>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key,
+                               src, dst, s->pk, s->grid, s->pr64, s->c64, e, vscale, MixedExtra<3>{s->mx, s->mc}, n);
+        else
+            hipLaunchKernelGGL((k_reorder<3, NoExtra>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key, src,
+                               dst, s->pk, s->grid, s->pr64, s->c64, e, vscale, NoExtra{}, n);
+    } else {
+        if (mixed)
+            hipLaunchKernelGGL((k_reorder<2, MixedExtra<2>>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key,
+                               src, dst, s->pk, s->grid, s->pr64, s->c64, e, vscale, MixedExtra<2>{s->mx, s->mc}, n);
+        else
+            hipLaunchKernelGGL((k_reorder<2, NoExtra>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key, src,
+                               dst, s->pk, s->grid, s->pr64, s->c64, e, vscale, NoExtra{}, n);
+    }
     s->cur = 1 - s->cur;
 }
 
@@ -219,7 +237,7 @@ static void launch_force64(wcsph_sim* s) {
     hipLaunchKernelGGL(k_force64<D>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
 }
 
-template <int D, int R, int Q, bool UH>
+template <int D, int R, int Q, bool UH, int VS>
 static void launch_nforce(wcsph_sim* s) {
     NFArgs a;
     a.pk = s->pk;
@@ -234,18 +252,69 @@ static void launch_nforce(wcsph_sim* s) {
     a.divv = s->divv;
     a.dti = s->dti;
     a.P = s->P;
-    hipLaunchKernelGGL((k_nforce<D, R, Q, UH>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
+    hipLaunchKernelGGL((k_nforce<D, R, Q, UH, VS>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc, 5);
+}
+
+template <int D, bool UH, int VS>
+static void nforce_dispatch_hv(wcsph_sim* s) {
+    if (s->R == 1) s->Q == 32 ? launch_nforce<D, 1, 32, UH, VS>(s) : launch_nforce<D, 1, 24, UH, VS>(s);
+    else s->Q == 32 ? launch_nforce<D, 2, 32, UH, VS>(s) : launch_nforce<D, 2, 24, UH, VS>(s);
 }
 
 template <int D, bool UH>
 static void nforce_dispatch_h(wcsph_sim* s) {
-    if (s->R == 1) s->Q == 32 ? launch_nforce<D, 1, 32, UH>(s) : launch_nforce<D, 1, 16, UH>(s);
-    else s->Q == 32 ? launch_nforce<D, 2, 32, UH>(s) : launch_nforce<D, 2, 16, UH>(s);
+    if (s->p.vsig_mode) nforce_dispatch_hv<D, UH, 1>(s);
+    else nforce_dispatch_hv<D, UH, 0>(s);
+}
+
+template <int D, int R, int Q>
+static void launch_nforce_mixed(wcsph_sim* s) {
+    NFMixedArgs a;
+    a.pk = s->pk;
+    a.mx = s->mx;
+    a.start = s->start;
+    a.g = s->grid;
+    a.n = s->n;
+    a.h = s->hmax;
+    a.inv_h = s->mc.inv_h;
+    a.Ls = (float)(s->grid.L / (double)s->hmax);
+    a.alpha = (float)s->p.alpha;
+    a.beta = (float)s->p.beta;
+    a.eps = (float)s->p.eps;
+    a.cnorm_s = (float)((D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI) / (D == 2 ? 8.0 : 16.0));
+    a.vsig_mode = s->p.vsig_mode;
+    auto splat = [](float v) { return h2{(_Float16)v, (_Float16)v}; };
+    a.k_eps = splat(a.eps);
+    a.k_cn = splat(a.cnorm_s);
+    a.k_malpha = splat(-2.f * a.alpha);
+    a.k_beta = splat(2.f * a.beta);
+    a.k_invh = splat(a.inv_h);
+    a.k_Ls = splat(a.Ls);
+    a.c0 = s->p.c0;
+    a.nu = s->p.nu;
+    a.c_cfl = s->p.c_cfl;
+    a.inv_rho0 = 1.0 / s->p.rho0;
+    a.acc = s->acc;
+    a.divv = s->divv;
+    a.dti = s->dti;
+    a.P = s->P;
+    hipLaunchKernelGGL((k_nforce_mixed<D, R, Q>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc, 5);
+}
+
+template <int D>
+static void nforce_mixed_dispatch(wcsph_sim* s) {
+    if (s->R == 1) s->Q == 32 ? launch_nforce_mixed<D, 1, 32>(s) : launch_nforce_mixed<D, 1, 24>(s);
+    else s->Q == 32 ? launch_nforce_mixed<D, 2, 32>(s) : launch_nforce_mixed<D, 2, 24>(s);
 }
 
 template <int D>
 static void nforce_dispatch(wcsph_sim* s) {
+    if (s->cfg.precision == WCSPH_PREC_MIXED) {
+        nforce_mixed_dispatch<D>(s);
+        return;
+    }
     if (s->uniform_h) nforce_dispatch_h<D, true>(s);
     else nforce_dispatch_h<D, false>(s);
 }
@@ -311,17 +380,19 @@ static void one_step(wcsph_sim* s) {
 }
 
 // FP64 diagnostics: per-block fixed-order tree sums, then one ordered pass.
-constexpr int kDiag = 9;
+constexpr int kDiag = 10;
 
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_diag(Master m, int64_t n, double g, double rho0, double B,
-                                                 double gamma, double zfront, double xcol, double* part) {
+                                                 double gamma, double zfront, double xcol, double xwall,
+                                                 double* part) {
     __shared__ double sh[kDiag][kBlock];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double v[kDiag];
     for (int k = 0; k < kDiag; ++k) v[k] = 0.0;
     v[7] = -1.0e300;
     v[8] = -1.0e300;
+    v[9] = -1.0e300;
     if (i < n) {
         const double mass = m.mass[i];
         double v2 = 0.0;
@@ -334,12 +405,14 @@ __global__ __launch_bounds__(kBlock) void k_diag(Master m, int64_t n, double g, 
         v[4] = 0.5 * mass * v2;
         const double z = m.pos[(D - 1) * n + i];
         v[5] = mass * g * z;
+        // internal energy of the clamped Tait law (de/drho = P/rho^2, zero below rho0)
         const double r = m.dens[i] / rho0;
-        v[6] = mass * B / rho0 * (pow(r, gamma - 1.0) / (gamma - 1.0) + 1.0 / r - gamma / (gamma - 1.0));
+        if (r > 1.0) v[6] = mass * B / rho0 * (pow(r, gamma - 1.0) / (gamma - 1.0) + 1.0 / r - gamma / (gamma - 1.0));
         if (m.type[i] == 0) {
             const double x = m.pos[i];
             if (z < zfront) v[7] = x;
             if (x < xcol) v[8] = z;
+            if (x > xwall) v[9] = z;
         }
     }
     for (int k = 0; k < kDiag; ++k) sh[k][threadIdx.x] = v[k];
@@ -347,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_diag(Master m, int64_t n, double g, 
     for (int off = kBlock / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) {
             for (int k = 0; k < 7; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
-            for (int k = 7; k < 9; ++k) sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + off]);
+            for (int k = 7; k < kDiag; ++k) sh[k][threadIdx.x] = fmax(sh[k][threadIdx.x], sh[k][threadIdx.x + off]);
         }
         __syncthreads();
     }
@@ -384,16 +457,14 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         WCSPH_REQUIRE(s->cfg.precision == WCSPH_PREC_FP32 || s->cfg.precision == WCSPH_PREC_FP64 ||
                           s->cfg.precision == WCSPH_PREC_MIXED,
                       WCSPH_EINVAL, "unknown precision");
-        WCSPH_REQUIRE(s->cfg.precision != WCSPH_PREC_MIXED, WCSPH_EINVAL,
-                      "mixed precision force path is not available in this build");
         s->D = p->dim;
         s->n = n;
         s->K = s->cfg.max_neighbours > 0 ? s->cfg.max_neighbours : 128;
         WCSPH_REQUIRE(s->K <= 4096, WCSPH_EINVAL, "max_neighbours above 4096");
         s->R = s->cfg.cells_per_support == 0 ? 2 : s->cfg.cells_per_support;
         WCSPH_REQUIRE(s->R == 1 || s->R == 2, WCSPH_EINVAL, "cells_per_support must be 1 or 2");
-        s->Q = s->cfg.queue_depth == 0 ? 16 : s->cfg.queue_depth;
-        WCSPH_REQUIRE(s->Q == 16 || s->Q == 32, WCSPH_EINVAL, "queue_depth must be 16 or 32");
+        s->Q = s->cfg.queue_depth == 0 ? 24 : s->cfg.queue_depth;
+        WCSPH_REQUIRE(s->Q == 24 || s->Q == 32, WCSPH_EINVAL, "queue_depth must be 24 or 32");
         // the fused pass packs a particle index and a row number in 32 bits
         WCSPH_REQUIRE(n < ((int64_t)1 << 27), WCSPH_EINVAL, "too many particles for one simulation (limit 2^27)");
         s->device = p->device;
@@ -422,6 +493,10 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         if (s->cfg.precision == WCSPH_PREC_FP64) {
             s->pr64 = s->alloc<double>(n);
             s->c64 = s->alloc<double>(n);
+        }
+        if (s->cfg.precision == WCSPH_PREC_MIXED) {
+            s->mx.sc16 = s->alloc<ScanRec16>((n + 1) / 2);
+            s->mx.q16 = s->alloc<Quad16>(n);
         }
         s->key = s->alloc<uint32_t>(n);
         s->slot = s->alloc<uint32_t>(n);
@@ -460,6 +535,17 @@ int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const d
             WCSPH_REQUIRE(type[i] == 0 || type[i] == 1, WCSPH_EINVAL, "type must be 0 (fluid) or 1 (wall)");
         }
         build_grid(s, pos, h);
+        if (s->cfg.precision == WCSPH_PREC_MIXED) {
+            WCSPH_REQUIRE(s->uniform_h, WCSPH_EINVAL, "the mixed-precision path needs one smoothing length for all particles");
+            WCSPH_REQUIRE(s->grid.nc[0] <= 2048, WCSPH_EINVAL,
+                          "the mixed-precision path needs at most 2048 cells along x (FP16 cell index)");
+            const double hh = (double)s->hmax;
+            s->mc.inv_h = (float)(1.0 / hh);
+            s->mc.inv_c0 = (float)(1.0 / s->p.c0);
+            s->mc.inv_rho0 = (float)(1.0 / s->p.rho0);
+            s->mc.pr_scale = (float)(s->p.rho0 / (s->p.c0 * s->p.c0));
+            s->mc.m_scale = (float)(1.0 / (s->p.rho0 * std::pow(hh, D)));
+        }
         std::vector<uint8_t> t8(n);
         std::vector<uint32_t> id(n);
         for (int64_t i = 0; i < n; ++i) {
@@ -491,13 +577,56 @@ int wcsph_sim_init(wcsph_sim* s) {
     });
 }
 
+// Two steps captured once per parity of the double-buffered state, then
+// replayed with one submission (the kernel arguments are fixed: the two
+// steps return the buffers to the parity they started from).
+static hipGraphExec_t two_step_graph(wcsph_sim* s) {
+    hipGraphExec_t& ge = s->graph2[s->cur];
+    if (ge) return ge;
+    const int cur0 = s->cur;
+    hipGraph_t g = nullptr;
+    WCSPH_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    one_step(s);
+    one_step(s);
+    WCSPH_HIP(hipStreamEndCapture(s->st, &g));
+    WCSPH_REQUIRE(s->cur == cur0, WCSPH_ESTATE, "two-step capture did not return to its parity");
+    const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    WCSPH_HIP(e);
+    return ge;
+}
+
 int wcsph_sim_step(wcsph_sim* s, int32_t nsteps) {
     return guarded([&] {
         WCSPH_REQUIRE(s && s->uploaded && s->forced, WCSPH_ESTATE, "call wcsph_sim_init first");
         WCSPH_REQUIRE(nsteps >= 0, WCSPH_EINVAL, "nsteps < 0");
         WCSPH_HIP(hipSetDevice(s->device));
-        for (int k = 0; k < nsteps; ++k) one_step(s);
+        int k = 0;
+        if (s->use_graphs && !s->timing) {
+            for (; k + 2 <= nsteps; k += 2) WCSPH_HIP(hipGraphLaunch(two_step_graph(s), s->st));
+        }
+        for (; k < nsteps; ++k) one_step(s);
         WCSPH_HIP(hipGetLastError());
+    });
+}
+
+int wcsph_sim_set_graphs(wcsph_sim* s, int32_t enable) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s, WCSPH_EINVAL, "null simulation");
+        s->use_graphs = enable != 0;
+    });
+}
+
+int wcsph_sim_query(wcsph_sim* s, int32_t* idle) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && idle, WCSPH_EINVAL, "null argument");
+        const hipError_t e = hipStreamQuery(s->st);
+        if (e == hipErrorNotReady) {
+            *idle = 0;
+            return;
+        }
+        WCSPH_HIP(e);
+        *idle = 1;
     });
 }
 
@@ -646,7 +775,7 @@ int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t 
     });
 }
 
-int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double* out) {
+int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double xwall, double* out) {
     return guarded([&] {
         WCSPH_REQUIRE(s && s->uploaded && out, WCSPH_EINVAL, "null argument or nothing uploaded");
         WCSPH_HIP(hipSetDevice(s->device));
@@ -655,10 +784,10 @@ int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double* out)
         const Master& m = s->m[s->cur];
         if (s->D == 3)
             hipLaunchKernelGGL(k_diag<3>, dim3(nblk), dim3(kBlock), 0, s->st, m, s->n, s->p.g, s->p.rho0, B,
-                               s->p.gamma, zfront, xcol, s->diag_part);
+                               s->p.gamma, zfront, xcol, xwall, s->diag_part);
         else
             hipLaunchKernelGGL(k_diag<2>, dim3(nblk), dim3(kBlock), 0, s->st, m, s->n, s->p.g, s->p.rho0, B,
-                               s->p.gamma, zfront, xcol, s->diag_part);
+                               s->p.gamma, zfront, xcol, xwall, s->diag_part);
         hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, s->st, s->diag_part, nblk, s->diag_out);
         WCSPH_HIP(hipGetLastError());
         WCSPH_HIP(hipMemcpyAsync(out, s->diag_out, sizeof(double) * kDiag, hipMemcpyDeviceToHost, s->st));

@@ -272,10 +272,10 @@ struct Packed {  // FP32 force records in cell order
 // Gather the master state into cell order and derive the force inputs
 // (CalcPressure, src/force.h:3-9 with Murnaghan, src/EoS.h:78-86).
 // pr64/c64 (P/rho^2 and c in FP64) are written only for the FP64 path.
-template <int D>
+template <int D, class Extra>
 __global__ void k_reorder(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ key, Master src,
                           Master dst, Packed pk, Grid g, double* __restrict__ pr64, double* __restrict__ c64,
-                          EosConsts e, int64_t n) {
+                          EosConsts e, float v_scale, Extra extra, int64_t n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const uint32_t o = perm[p];
@@ -291,7 +291,7 @@ __global__ void k_reorder(const uint32_t* __restrict__ perm, const uint32_t* __r
         dst.vel[d * n + p] = v;
         dst.vh[d * n + p] = src.vh[d * n + o];
         xl[d] = cell_local(x, g.lo[d], c[d], g.L);
-        vf[d] = (float)v;
+        vf[d] = (float)(v * v_scale);
     }
     const double rho = src.dens[o], m = src.mass[o], h = src.h[o];
     dst.dens[p] = rho;
@@ -313,7 +313,12 @@ __global__ void k_reorder(const uint32_t* __restrict__ perm, const uint32_t* __r
         pr64[p] = pr;
         c64[p] = cs;
     }
+    extra(p, xl, c[0], rho, pr, cs, m);
 }
+
+struct NoExtra {
+    __device__ void operator()(int64_t, const float*, int, double, double, double, double) const {}
+};
 
 // Walks the (2R+1)^D cells around particle i's cell (R = cells per
 // support length), yielding the candidate index range of each non-empty
@@ -351,25 +356,28 @@ struct Stencil {
             if (p < pend) return;
         }
     }
-    // FP32 offset (c_i - c_j) L for the current cell, per axis
-    __device__ __forceinline__ static float off(int kk, int axis, float Lf) {
+    // (c_i - c_j) for the cell of stencil slot kk, per axis, as a float
+    __device__ __forceinline__ static float dcell(int kk, int axis) {
         const int d = axis == 0 ? kk % S - R : axis == 1 ? (kk / S) % S - R : kk / (S * S) - R;
-        return (float)(-d) * Lf;
+        return (float)(-d);
     }
 };
 
-// The neighbour predicate on cell-local coordinates (no contraction):
-// d = (xl_i - xl_j) + (c_i - c_j) L per axis, r2 = sum d*d, r2 < (h_i + h_j)^2.
+// The neighbour predicate on cell-local coordinates: per axis
+// d = fma(c_i - c_j, L, xl_i) - xl_j, r2 = fma(dz, dz, fma(dy, dy, dx dx)),
+// neighbour iff 0 < r2 < (h_i + h_j)^2; every operation is written out so
+// nothing else is contracted (the oracle evaluates the same sequence).
+// dc: (c_i - c_j) per axis as exact small floats.
 template <int D>
-__device__ __forceinline__ bool nbr_pred(const float* xi, float hi, const float4& pj, const float* off, float hj) {
+__device__ __forceinline__ bool nbr_pred(const float* xi, float hi, const float4& pj, const float* dc, float Lf,
+                                         float hj) {
     const float pjv[3] = {pj.x, pj.y, pj.z};
-    float r2 = 0.f;
+    float d[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        const float t = __fsub_rn(xi[d], pjv[d]);
-        const float dd = __fadd_rn(t, off[d]);
-        r2 = __fadd_rn(r2, __fmul_rn(dd, dd));
-    }
+    for (int k = 0; k < D; ++k) d[k] = __fsub_rn(fmaf(dc[k], Lf, xi[k]), pjv[k]);
+    float r2 = __fmul_rn(d[0], d[0]);
+    r2 = fmaf(d[1], d[1], r2);
+    if (D == 3) r2 = fmaf(d[2], d[2], r2);
     const float H = __fadd_rn(hi, hj);
     return r2 > 0.f && r2 < __fmul_rn(H, H);
 }
@@ -471,11 +479,11 @@ __global__ __launch_bounds__(kBlock) void k_neigh(Packed pk, const uint32_t* __r
         st.k = -1;
         st.advance(g, start, c, xi, hc * hc);
         while (st.k < Stencil<D, R>::NS) {
-            float off[3];
+            float dc[3];
 #pragma unroll
-            for (int d = 0; d < D; ++d) off[d] = Stencil<D, R>::off(st.k, d, g.Lf);
+            for (int d = 0; d < D; ++d) dc[d] = Stencil<D, R>::dcell(st.k, d);
             for (uint32_t j = st.p; j < st.pend; ++j) {
-                if (nbr_pred<D>(xi, hi, pk.px[j], off, pk.pq[j].w)) {
+                if (nbr_pred<D>(xi, hi, pk.px[j], dc, g.Lf, pk.pq[j].w)) {
                     if ((int)c_out < K) nbr[(int64_t)c_out * n + i] = j;
                     ++c_out;
                 }
@@ -513,21 +521,28 @@ __device__ __forceinline__ float axis_gap(int d, float x, float Lf) {
     return fmaxf(fmaxf(lo - x, x - hi), 0.f);
 }
 
+// Wave-wide maximum, returned as a wave-uniform (scalar) value.
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) {
         const uint32_t w = __shfl_xor(v, o, 64);
         v = w > v ? w : v;
     }
-    return v;
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
-template <int D, int R, int Q, bool UH>
+constexpr int kScanUnroll = 4;  // candidate loads in flight per lane
+
+template <int D, int R, int Q, bool UH, int VS>
 __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
     constexpr int S = 2 * R + 1;
     constexpr int NR = D == 3 ? S * S : S;  // rows of cells along x
     constexpr int kShift = 27;
     constexpr uint32_t kMask = (1u << kShift) - 1u;
     __shared__ uint32_t qbuf[kBlock / 64][64 * (Q + 1)];
+    __shared__ float2 rowdc[NR];  // (c_i - c_j) along y and z for each row
+    if ((int)threadIdx.x < NR)
+        rowdc[threadIdx.x] = make_float2((float)(R - (int)threadIdx.x % S), D == 3 ? (float)(R - (int)threadIdx.x / S) : 0.f);
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     uint32_t* q = &qbuf[threadIdx.x >> 6][lane * (Q + 1)];
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
@@ -544,6 +559,14 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
     if (live) key_cell<D>(a.g, a.pk.pcell[i], c);
     const float hc = (UH ? 2.f * I.h : I.h + a.hmax) * 1.0001f;
     const float hc2 = hc * hc;
+    UniformH uh;
+    if (UH) {
+        uh.h = I.h;
+        uh.eps_h2 = a.k.eps * I.h * I.h;
+        uh.invH = 0.5f / I.h;
+        const float iH2 = uh.invH * uh.invH;
+        uh.invHD1 = D == 3 ? iH2 * iH2 : iH2 * uh.invH;
+    }
     float acc[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) acc[d] = 0.f;
@@ -551,8 +574,7 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
     uint32_t qn = 0, total = 0;
 
     auto flush = [&]() {
-        const uint32_t qmax = wave_max(qn);
-        for (uint32_t t = 0; t < qmax; ++t) {
+        for (uint32_t t = 0; __any(t < qn); ++t) {
             if (t < qn) {
                 const uint32_t e = q[t];
                 const uint32_t j = e & kMask;
@@ -560,33 +582,38 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
                 const float4 pj = a.pk.px[j];
                 const float4 vj = a.pk.pv[j];
                 const float4 qj = a.pk.pq[j];
-                const float off[3] = {(cxi - pj.w) * Lf, (float)(R - r % S) * Lf,
-                                      D == 3 ? (float)(R - r / S) * Lf : 0.f};
+                const float2 dyz = rowdc[r];
+                const float dc[3] = {cxi - pj.w, dyz.x, dyz.y};
                 const float pjv[3] = {pj.x, pj.y, pj.z};
                 const float vjv[3] = {vj.x, vj.y, vj.z};
                 float dr[D], dv[D];
                 float r2 = 0.f, drdv = 0.f;
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    dr[d] = (xi[d] - pjv[d]) + off[d];
+                    dr[d] = fmaf(dc[d], Lf, xi[d]) - pjv[d];
                     dv[d] = vi[d] - vjv[d];
                     r2 += dr[d] * dr[d];
                     drdv += dr[d] * dv[d];
                 }
                 const JState<float> J{qj.w, qj.x, qj.y, qj.z, a.nu, vj.w};
-                float mu;
-                const float den = pair_mu(r2, drdv, I, J.h, a.k, mu, vsig);
-                pair_force<D>(dr, dv, r2, drdv, den, mu, I, J, a.k, acc, divv);
+                if (UH) {
+                    pair_uniform<D, VS>(dr, dv, r2, drdv, I, J, a.k, uh, acc, divv, vsig);
+                } else {
+                    float mu;
+                    const float den = pair_mu(r2, drdv, I, J.h, a.k, mu, vsig);
+                    pair_force<D>(dr, dv, r2, drdv, den, mu, I, J, a.k, acc, divv);
+                }
             }
         }
         total += qn;
         qn = 0;
     };
 
-    for (int r = 0; r < NR; ++r) {
+    // candidate range of row r for this lane: indices [start[b0], start[b1])
+    auto row_bounds = [&](int r, uint32_t& b0, uint32_t& b1) {
         const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
-        uint32_t p0 = 0, len = 0;
         const int cy = c[1] + dy, cz = c[2] + dz;
+        b0 = b1 = 0;
         if (live && cy >= 0 && cy < a.g.nc[1] && (D == 2 || (cz >= 0 && cz < a.g.nc[2]))) {
             const float gy = axis_gap(dy, xi[1], Lf);
             const float gz = D == 3 ? axis_gap(dz, xi[2], Lf) : 0.f;
@@ -599,25 +626,55 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
                 if (xa <= xb) {
                     const uint32_t row = D == 3 ? (uint32_t)((cz * a.g.nc[1] + cy) * a.g.nc[0])
                                                 : (uint32_t)(cy * a.g.nc[0]);
-                    p0 = a.start[row + c[0] + xa];
-                    len = a.start[row + c[0] + xb + 1] - p0;
+                    b0 = row + c[0] + xa;
+                    b1 = row + c[0] + xb + 1;
                 }
             }
         }
-        const float offy = (float)(-dy) * Lf, offz = (float)(-dz) * Lf;
-        const uint32_t tmax = wave_max(len);
-        for (uint32_t t = 0; t < tmax; ++t) {
-            if (t < len) {
-                const uint32_t j = p0 + t;
-                const float4 pj = a.pk.px[j];
-                const float hj = UH ? I.h : a.pk.pq[j].w;
-                const float off[3] = {(cxi - pj.w) * Lf, offy, offz};
-                if (nbr_pred<D>(xi, I.h, pj, off, hj)) {
-                    q[qn] = j | ((uint32_t)r << kShift);
+    };
+    uint32_t nb0, nb1;
+    row_bounds(0, nb0, nb1);
+    uint32_t np0 = nb1 ? a.start[nb0] : 0u, npe = nb1 ? a.start[nb1] : 0u;
+    for (int r = 0; r < NR; ++r) {
+        const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
+        const uint32_t p0 = np0, len = npe - np0;
+        if (r + 1 < NR) {  // prefetch the next row's range behind this row's scan
+            row_bounds(r + 1, nb0, nb1);
+            np0 = nb1 ? a.start[nb0] : 0u;
+            npe = nb1 ? a.start[nb1] : 0u;
+        }
+        // the y/z part of the predicate is fixed for the row
+        const float yi = fmaf((float)(-dy), Lf, xi[1]);
+        const float zi = D == 3 ? fmaf((float)(-dz), Lf, xi[2]) : 0.f;
+        const float HH = UH ? __fmul_rn(__fadd_rn(I.h, I.h), __fadd_rn(I.h, I.h)) : 0.f;
+        for (uint32_t t = 0; __any(t < len); t += kScanUnroll) {
+            // issue all loads of the group before testing any of them
+            float4 pj[kScanUnroll];
+            float hj[kScanUnroll];
+#pragma unroll
+            for (int u = 0; u < kScanUnroll; ++u) {
+                const uint32_t j = p0 + min(t + u, len ? len - 1 : 0u);
+                pj[u] = a.pk.px[j];
+                hj[u] = UH ? I.h : a.pk.pq[j].w;
+            }
+#pragma unroll
+            for (int u = 0; u < kScanUnroll; ++u) {
+                const float dx = __fsub_rn(fmaf(cxi - pj[u].w, Lf, xi[0]), pj[u].x);
+                const float dy = __fsub_rn(yi, pj[u].y);
+                float r2 = __fmul_rn(dx, dx);
+                r2 = fmaf(dy, dy, r2);
+                if (D == 3) r2 = fmaf(__fsub_rn(zi, pj[u].z), __fsub_rn(zi, pj[u].z), r2);
+                float H2 = HH;
+                if (!UH) {
+                    const float H = __fadd_rn(I.h, hj[u]);
+                    H2 = __fmul_rn(H, H);
+                }
+                if (t + u < len && r2 > 0.f && r2 < H2) {
+                    q[qn] = (p0 + t + u) | ((uint32_t)r << kShift);
                     ++qn;
                 }
             }
-            if (__any(qn == (uint32_t)Q)) flush();
+            if (__any(qn + kScanUnroll > (uint32_t)Q)) flush();
         }
     }
     flush();
