@@ -117,6 +117,39 @@ def accuracy_study(W, dim, dx, marks, chunk, early=100):
     return out
 
 
+def serving_study(W, seed, count, rate, max_slots, chunk, precision, budget_gib):
+    """Continuous-batching run of a seeded Poisson request stream against the
+    same stream served one request at a time (max_slots = 1)."""
+    import numpy as np
+    reqs = W.poisson_workload(seed, count, rate, precision=precision)
+    runs, results = {}, []
+    for slots in (max_slots, 1):
+        t0 = time.perf_counter()
+        res = W.serve(reqs, hbm_budget_bytes=int(budget_gib * (1 << 30)), max_slots=slots, chunk_steps=chunk)
+        wall = time.perf_counter() - t0
+        lat = np.array([r["latency_s"] for r in res])
+        pairs = sum(r["pairs"] for r in res)
+        makespan = max(r["end_s"] for r in res)
+        runs[f"slots_{slots}"] = {
+            "makespan_s": makespan, "wall_s": wall,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "queue_wait_p50_s": float(np.median([r["start_s"] - r["arrival_s"] for r in res])),
+            "pair_interactions": pairs, "pairs_per_s": pairs / makespan,
+            "status": sorted({r["status"] for r in res}),
+            "fp32_fallbacks": sum(r["precision_used"] != reqs[k].precision for k, r in enumerate(res)),
+        }
+        results.append(res)
+        print(f"# slots {slots}: makespan {makespan:.2f} s", file=sys.stderr, flush=True)
+    keys = ("steps", "pairs", "sim_time", "front", "column", "kinetic")
+    identical = all(a[k] == b[k] for a, b in zip(*results) for k in keys)
+    return {"workload": f"{count} dam-break requests, Poisson arrivals at {rate}/s (seed {seed}), 2D/3D, "
+                        f"20k-2M particles, 500-5000 steps, {precision}",
+            "requests": [{"dim": r.geom.dim, "dx": r.geom.dx, "steps": r.steps, "arrival_s": r.arrival_s}
+                         for r in reqs],
+            "max_slots": max_slots, "chunk_steps": chunk, "hbm_budget_gib": budget_gib, "runs": runs,
+            "results_identical_batched_vs_one_at_a_time": identical}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +165,12 @@ def main():
     ap.add_argument("--accuracy", action="store_true",
                     help="run the FP32 and mixed paths against the FP64 path to --t-end and print their errors")
     ap.add_argument("--t-end", type=float, default=1.0)
+    ap.add_argument("--serve", action="store_true",
+                    help="run a seeded request stream through the serving loop (p50/p99 latency, makespan)")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-seed", type=int, default=2026)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -142,6 +181,11 @@ def main():
 
     dim = args.dim
     dx = args.dx or (0.0126 if dim == 3 else 0.02)
+    if args.serve:
+        prec = args.precision if args.precision != "fp32" or "--precision" in sys.argv else "mixed"
+        print(json.dumps(serving_study(W, args.serve_seed, args.serve_requests, args.serve_rate, args.serve_slots,
+                                       50, prec, 64.0)), flush=True)
+        return
     if args.accuracy:
         marks = [m for m in (0.25, 0.5, 1.0) if m <= args.t_end] or [args.t_end]
         print(json.dumps(accuracy_study(W, dim, dx, marks, 50)), flush=True)
