@@ -1,21 +1,33 @@
 // Mixed-precision fused neighbour + force pass (uniform smoothing length).
 //
 // Units: lengths in h, velocities in c0, densities in rho0, P/rho^2 in
-// c0^2/rho0, masses in rho0 h^D. In these units every quantity of the pair
-// body is O(1e-4..50) and fits FP16 (in SI units 1/H^4 alone would
-// overflow it).
+// c0^2/rho0, masses in rho0 h^D (times the kernel constant C_D/2^(D+1)).
+// In these units every quantity of the pair body is O(1e-4..50) and fits
+// FP16 (in SI units 1/H^4 alone would overflow it).
 //
-//  * scan: particles are stored in pairs (2m, 2m+1) as half2 lanes of one
-//    16-byte record {x, y, z, x-cell}, cell-local and h-scaled, so one load
-//    and one packed FP16 predicate test two candidates;
-//  * body: two queued neighbours per lane step; pair differences are formed
-//    in FP32 from the FP32 cell-local records, scaled, rounded once to FP16
-//    (v_cvt_pk_f16_f32), and the whole pair body runs on packed FP16 VALU
-//    ops (v_pk_fma_f16 ...); per-particle sums accumulate in FP32 through
-//    v_dot2_f32_f16.
-// The neighbour set of this path is its own FP16 predicate's: it differs from
-// the FP32 path's only for pairs within ~1e-3 h of the support edge, whose
-// kernel weight is ~1e-6.
+// Pair records: particles 2m and 2m+1 (cell order) share one 16-byte record
+// of each kind, as the two halves of packed half2 values:
+//   PosRec16   {x, y, z, x-cell}   position relative to the cell centre / h
+//                                  (|x| <= L/2h, where FP16 has its finest
+//                                  steps) and the cell's x index
+//   VelRec16   {vx, vy, vz, m}     v / c0 and the scaled mass
+//   StateRec16 {rho, pr, c/2, -}   rho / rho0, (P / rho^2)(rho0 / c0^2), c / (2 c0)
+// One 16-byte load therefore brings two candidates, and every pair-body
+// operation is one packed FP16 VALU op (v_pk_fma_f16, v_pk_mul_f16 ...) for
+// two pairs. Pair differences are taken in FP16 from the h-scaled cell-local
+// coordinates, the small difference first and the whole-cell offset last:
+// dx = (x_i - x_j) + (cx_i - cx_j) L/h, dy = (y_i - y_j) - dy_row L/h, so only
+// the last rounding is at the scale of the result. The scan's predicate uses
+// the cheaper per-row form (y_i - dy_row L/h) - y_j. Per-particle sums
+// accumulate in FP32 through v_dot2_f32_f16.
+//
+// Scan: per row of cells each lane walks the records of its chord-clipped
+// window; a record with at least one neighbour half goes into the lane's
+// LDS ring queue as (record | row << 25 | hit bits << 30). The queue is
+// drained K entries per lane at a time, so every lane of the wave has work
+// for most of a drain. The neighbour set is this path's FP16 predicate
+// 0 < r^2 < 4 h^2: it differs from the FP32 path's only for pairs within
+// ~1e-3 h of the support edge, whose kernel weight is ~1e-6.
 #pragma once
 #include "sim_kernels.hpp"
 
@@ -23,47 +35,59 @@ namespace wcsph {
 
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
-struct alignas(16) ScanRec16 {  // particles 2m (.x) and 2m+1 (.y)
+struct alignas(16) PosRec16 {
     h2 x, y, z, cx;
 };
-
-struct alignas(8) Quad16 {  // per particle
-    _Float16 rho, pr, c, m;
+struct alignas(16) VelRec16 {
+    h2 vx, vy, vz, m;
+};
+struct alignas(16) StateRec16 {
+    h2 rho, pr, ch, pad;
 };
 
 struct Mixed {
-    ScanRec16* sc16;  // [(n+1)/2]
-    Quad16* q16;      // [n]
+    PosRec16* pos;  // [(n + 1) / 2]
+    VelRec16* vel;
+    StateRec16* st;
 };
+
+__host__ __device__ inline int64_t mixed_records(int64_t n) { return (n + 1) / 2; }
 
 struct MixedConsts {
-    float inv_h, inv_c0, inv_rho0, pr_scale, m_scale;  // SI -> scaled units
+    float inv_h, inv_c0, inv_rho0, pr_scale, m_scale;  // SI -> scaled units; m_scale includes C_D / 2^(D+1)
+    float half_Ls;  // half the cell edge in h: positions are stored relative to the cell centre
 };
 
-// Extra stores of k_reorder in mixed mode (thread p owns half p&1 of record p>>1).
+// Extra stores of k_reorder in mixed mode: thread p owns half p & 1 of
+// record p >> 1.
 template <int D>
 __device__ __forceinline__ void write_mixed(const Mixed& mx, const MixedConsts& mc, int64_t p, const float* xl,
-                                            int cx, double rho, double pr, double c, double m) {
-    _Float16* r = reinterpret_cast<_Float16*>(&mx.sc16[p >> 1]);
+                                            int cx, const float* vf, double rho, double pr, double c, double m) {
     const int o = (int)(p & 1);
-    r[0 + o] = (_Float16)(xl[0] * mc.inv_h);
-    r[2 + o] = (_Float16)(xl[1] * mc.inv_h);
-    r[4 + o] = (_Float16)(D == 3 ? xl[2] * mc.inv_h : 0.f);
-    r[6 + o] = (_Float16)(float)cx;
-    Quad16 q;
-    q.rho = (_Float16)(float)(rho * mc.inv_rho0);
-    q.pr = (_Float16)(float)(pr * mc.pr_scale);
-    q.c = (_Float16)(float)(c * mc.inv_c0);
-    q.m = (_Float16)(float)(m * mc.m_scale);
-    mx.q16[p] = q;
+    _Float16* P = reinterpret_cast<_Float16*>(&mx.pos[p >> 1]) + o;
+    _Float16* V = reinterpret_cast<_Float16*>(&mx.vel[p >> 1]) + o;
+    _Float16* S = reinterpret_cast<_Float16*>(&mx.st[p >> 1]) + o;
+    P[0] = (_Float16)(xl[0] * mc.inv_h - mc.half_Ls);
+    P[2] = (_Float16)(xl[1] * mc.inv_h - mc.half_Ls);
+    P[4] = (_Float16)(D == 3 ? xl[2] * mc.inv_h - mc.half_Ls : 0.f);
+    P[6] = (_Float16)(float)cx;
+    V[0] = (_Float16)vf[0];
+    V[2] = (_Float16)vf[1];
+    V[4] = (_Float16)(D == 3 ? vf[2] : 0.f);
+    V[6] = (_Float16)(float)(m * mc.m_scale);
+    S[0] = (_Float16)(float)(rho * mc.inv_rho0);
+    S[2] = (_Float16)(float)(pr * mc.pr_scale);
+    S[4] = (_Float16)(float)(0.5 * c * mc.inv_c0);
+    S[6] = (_Float16)0.f;
 }
 
 template <int D>
-struct MixedExtra {  // k_reorder hook writing the FP16 records
+struct MixedExtra {  // k_reorder hook writing the pair records
     Mixed mx;
     MixedConsts mc;
-    __device__ void operator()(int64_t p, const float* xl, int cx, double rho, double pr, double c, double m) const {
-        write_mixed<D>(mx, mc, p, xl, cx, rho, pr, c, m);
+    __device__ void operator()(int64_t p, const float* xl, int cx, const float* vf, double rho, double pr, double c,
+                               double m) const {
+        write_mixed<D>(mx, mc, p, xl, cx, vf, rho, pr, c, m);
     }
 };
 
@@ -71,7 +95,6 @@ __device__ __forceinline__ h2 hsplat(float v) {
     const _Float16 h = (_Float16)v;
     return h2{h, h};
 }
-__device__ __forceinline__ h2 hpack(float a, float b) { return h2{(_Float16)a, (_Float16)b}; }
 __device__ __forceinline__ h2 hrcp(h2 v) {
     return h2{(_Float16)__builtin_amdgcn_rcph(v.x), (_Float16)__builtin_amdgcn_rcph(v.y)};
 }
@@ -80,18 +103,22 @@ __device__ __forceinline__ h2 hrsq(h2 v) {
 }
 __device__ __forceinline__ h2 hmax(h2 a, h2 b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ h2 hmin(h2 a, h2 b) { return __builtin_elementwise_min(a, b); }
+__device__ __forceinline__ h2 hfma(h2 a, h2 b, h2 c) { return __builtin_elementwise_fma(a, b, c); }
+__device__ __forceinline__ h2 as_h2(uint32_t u) { return __builtin_bit_cast(h2, u); }
+__device__ __forceinline__ uint32_t as_u32(h2 v) { return __builtin_bit_cast(uint32_t, v); }
 
 struct NFMixedArgs {
-    Packed pk;  // px: FP32 cell-local (SI) + x cell; pv: v / c0 (FP32) + m
     Mixed mx;
+    const float4* px;  // FP32 cell-local positions (row windows of i)
+    const float4* pq;  // FP32 rho, P/rho^2, c, h (outputs of i)
+    const uint32_t* pcell;
     const uint32_t* start;
     Grid g;
     int64_t n;
-    float h, inv_h, Ls;  // uniform smoothing length, its inverse (as in the records); cell edge / h
-    float alpha, beta, eps, cnorm_s;  // cnorm_s = C_D / 2^(D+1)
-    int vsig_mode;
+    int64_t nrec;  // pair records
+    float h;
     // wave-uniform packed constants (kept in scalar registers)
-    h2 k_eps, k_cn, k_malpha, k_beta, k_invh, k_Ls;
+    h2 k_eps, k_malpha, k_beta, k_Ls;
     double c0, nu, c_cfl, inv_rho0;
     double* acc;
     double* divv;
@@ -99,96 +126,152 @@ struct NFMixedArgs {
     Partials P;
 };
 
-template <int D, int R, int Q>
-__global__ __launch_bounds__(kBlock) void k_nforce_mixed(NFMixedArgs a) {
+// Tuning constants (build-time overridable): ring queue entries per lane
+// (a power of two), entries drained per lane per drain, scan records in
+// flight per lane, and the waves per SIMD the register budget is fitted to.
+#ifndef WCSPH_MIX_Q
+#define WCSPH_MIX_Q 16
+#endif
+#ifndef WCSPH_MIX_K
+#define WCSPH_MIX_K 8
+#endif
+#ifndef WCSPH_MIX_U
+#define WCSPH_MIX_U 2
+#endif
+#ifndef WCSPH_MIX_WAVES
+#define WCSPH_MIX_WAVES 4
+#endif
+constexpr int kMixQ = WCSPH_MIX_Q;
+constexpr int kMixK = WCSPH_MIX_K;
+constexpr int kMixU = WCSPH_MIX_U;
+constexpr int kMixRecBits = 25;  // record index bits of a queue entry
+static_assert((kMixQ & (kMixQ - 1)) == 0 && kMixK <= kMixQ && kMixU < kMixQ, "queue shape");
+
+// Buffer resource over n bytes at p for range-checked loads: a raw buffer
+// load whose offset lies beyond n returns zeros instead of touching memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, int32_t n) {
+    constexpr int32_t kGfx9RawBuffer = 0x00020000;  // dword 3 of a raw (untyped) buffer on gfx9-family parts
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, n, kGfx9RawBuffer);
+}
+
+template <int D, int R, int VS, int BT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WCSPH_MIX_WAVES)))
+void k_nforce_mixed(NFMixedArgs a) {
     constexpr int S = 2 * R + 1;
     constexpr int NR = D == 3 ? S * S : S;
-    constexpr int kShift = 27;
-    constexpr uint32_t kMask = (1u << kShift) - 1u;
-    __shared__ uint32_t qbuf[kBlock / 64][64 * (Q + 1)];
-    __shared__ float2 rowdc[NR];
-    if ((int)threadIdx.x < NR)
-        rowdc[threadIdx.x] = make_float2((float)(R - (int)threadIdx.x % S), D == 3 ? (float)(R - (int)threadIdx.x / S) : 0.f);
+    constexpr int Q = kMixQ, K = kMixK, U = kMixU;
+    constexpr uint32_t kRecMask = (1u << kMixRecBits) - 1u;
+    static_assert(NR <= 32, "row index of a queue entry has 5 bits");
+    // slot-major ring queues: slot t of lane l at [t][l], so lanes reading
+    // different slots still hit different banks
+    __shared__ uint32_t qbuf[kBlock / 64][Q * 64];
+    // per (row, hit bits) of a queue entry: the row's (dy L/h, dz L/h) as
+    // half2 splats, the half2 mask keeping the hit halves, the hit count
+    __shared__ uint4 etab[128];
+    if (threadIdx.x < 128) {
+        const int row = (int)threadIdx.x & 31, bits = (int)threadIdx.x >> 5;
+        const h2 dy = hsplat((float)(row % S - R)) * a.k_Ls;
+        const h2 dz = hsplat(D == 3 ? (float)(row / S - R) : 0.f) * a.k_Ls;
+        const uint32_t sel = ((bits & 1) ? 0x0000ffffu : 0u) | ((bits & 2) ? 0xffff0000u : 0u);
+        etab[threadIdx.x] = make_uint4(as_u32(dy), as_u32(dz), sel, (uint32_t)__builtin_popcount(bits));
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    uint32_t* q = &qbuf[threadIdx.x >> 6][lane * (Q + 1)];
+    uint32_t* q = &qbuf[threadIdx.x >> 6][lane];
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     const bool live = i < a.n;
-    const float4 xi4 = live ? a.pk.px[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 vi4 = live ? a.pk.pv[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    Quad16 qi;
-    if (live) qi = a.mx.q16[i];
-    else qi = Quad16{(_Float16)1.f, (_Float16)0.f, (_Float16)1.f, (_Float16)0.f};
-    const float xi[3] = {xi4.x, xi4.y, xi4.z};
-    const float vi[3] = {vi4.x, vi4.y, vi4.z};
-    const float Lf = a.g.Lf, invLf = 1.0f / Lf, inv_h = a.inv_h;
-    const float xs[3] = {xi[0] * inv_h, xi[1] * inv_h, xi[2] * inv_h};
+    const int64_t ii = live ? i : 0;
+    const float4 xf = a.px[ii];
+    const float xi[3] = {xf.x, xf.y, xf.z};
+    const bool odd = (ii & 1) != 0;
+    const PosRec16 Pi = a.mx.pos[ii >> 1];
+    const VelRec16 Vi = a.mx.vel[ii >> 1];
+    const StateRec16 Si = a.mx.st[ii >> 1];
+    auto pick = [&](h2 v) {
+        const _Float16 s = odd ? v.y : v.x;
+        return h2{s, s};
+    };
+    const h2 xi2 = pick(Pi.x), yi2 = pick(Pi.y), zi2 = pick(Pi.z), cxi2 = pick(Pi.cx);
+    const h2 vxi = pick(Vi.vx), vyi = pick(Vi.vy), vzi = pick(Vi.vz);
+    const h2 rhoi = pick(Si.rho), pri = pick(Si.pr), chi = pick(Si.ch);
+    const h2 Ls2 = a.k_Ls, eps2 = a.k_eps, malpha2 = a.k_malpha, beta2 = a.k_beta;
+    const h2 zero2 = hsplat(0.f), tiny2 = hsplat(1.0e-6f), sixteen2 = hsplat(16.f);
+    const _Float16 four = (_Float16)4.f;
+    const float Lf = a.g.Lf, invLf = 1.0f / Lf;
     int c[3] = {0, 0, 0};
-    if (live) key_cell<D>(a.g, a.pk.pcell[i], c);
+    key_cell<D>(a.g, a.pcell[ii], c);
     const float hc = 2.f * a.h * 1.0001f, hc2 = hc * hc;
-    const _Float16 H2s = (_Float16)4.0f;  // (2h)^2 in units of h^2
-    const h2 cxi2 = hsplat((float)c[0]), xi2 = hsplat(xs[0]), Ls2 = a.k_Ls;
-    const h2 rhoi2 = h2{qi.rho, qi.rho}, pri2 = h2{qi.pr, qi.pr}, ci2 = h2{qi.c, qi.c};
-    const h2 eps2 = a.k_eps, half2c = h2{(_Float16)0.5f, (_Float16)0.5f}, one2 = h2{(_Float16)1.f, (_Float16)1.f};
-    const h2 three2 = h2{(_Float16)-3.f, (_Float16)-3.f}, twelve2 = h2{(_Float16)12.f, (_Float16)12.f};
-    const h2 cn2 = a.k_cn, malpha2 = a.k_malpha, beta2 = a.k_beta, inv_h2 = a.k_invh;
     float accp[3] = {0.f, 0.f, 0.f}, accv[3] = {0.f, 0.f, 0.f}, divv = 0.f;
-    h2 vsig2 = hsplat(0.f);
-    uint32_t qn = 0, total = 0;
+    h2 vsig2 = zero2;
+    uint32_t qh = 0, qn = 0, total = 0;
+    const __amdgpu_buffer_rsrc_t pos_rsrc = buffer_rsrc(a.mx.pos, (int32_t)(a.nrec * (int64_t)sizeof(PosRec16)));
 
-    auto flush = [&]() {
-        for (uint32_t t = 0; __any(t < qn); t += 2) {
-            if (t < qn) {
-                const bool two = t + 1 < qn;
-                const uint32_t e0 = q[t], e1 = two ? q[t + 1] : e0;
-                const uint32_t j0 = e0 & kMask, j1 = e1 & kMask;
-                const float4 p0 = a.pk.px[j0], p1 = a.pk.px[j1];
-                const float4 v0 = a.pk.pv[j0], v1 = a.pk.pv[j1];
-                const Quad16 q0 = a.mx.q16[j0], q1 = a.mx.q16[j1];
-                const float2 d0 = rowdc[e0 >> kShift], d1 = rowdc[e1 >> kShift];
-                const float dc0[3] = {xi4.w - p0.w, d0.x, d0.y}, dc1[3] = {xi4.w - p1.w, d1.x, d1.y};
-                const float pj0[3] = {p0.x, p0.y, p0.z}, pj1[3] = {p1.x, p1.y, p1.z};
-                const float vj0[3] = {v0.x, v0.y, v0.z}, vj1[3] = {v1.x, v1.y, v1.z};
-                h2 dr[D], dv[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    dr[d] = hpack(fmaf(dc0[d], Lf, xi[d]) - pj0[d], fmaf(dc1[d], Lf, xi[d]) - pj1[d]) * inv_h2;
-                    dv[d] = hpack(vi[d] - vj0[d], vi[d] - vj1[d]);
-                }
-                h2 r2 = dr[0] * dr[0], drdv = dr[0] * dv[0];
-#pragma unroll
-                for (int d = 1; d < D; ++d) {
-                    r2 = dr[d] * dr[d] + r2;
-                    drdv = dr[d] * dv[d] + drdv;
-                }
-                r2 = hmax(r2, hsplat(1.0e-6f));  // FP16 underflow guard (kernel weight ~0 there)
-                const h2 rhoj = h2{q0.rho, q1.rho}, prj = h2{q0.pr, q1.pr};
-                const h2 cj = h2{q0.c, q1.c}, mj = h2{q0.m, q1.m * (_Float16)(two ? 1.f : 0.f)};
-                const h2 den = r2 + eps2;                     // r^2 + eps h_i h_j
-                const h2 rho_s = rhoi2 + rhoj;
-                const h2 X = hrcp(den * rho_s);               // 1 / (den (rho_i + rho_j))
-                const h2 inv_den = X * rho_s, inv_rs = X * den;
-                const h2 mu = hmin(drdv, hsplat(0.f)) * inv_den;  // (h_i + h_j)/2 = 1
-                if (a.vsig_mode) vsig2 = hmax(vsig2, -mu);
-                const h2 cbar = (ci2 + cj) * half2c;
-                const h2 Pi = (malpha2 * cbar + beta2 * mu) * mu * inv_rs;
-                const h2 invr = hrsq(r2);
-                const h2 sx = r2 * invr * half2c;             // r / (2h)
-                const h2 aa = hmax(one2 - sx, hsplat(0.f)), bb = hmax(half2c - sx, hsplat(0.f));
-                const h2 g = three2 * aa * aa + twelve2 * bb * bb;
-                const h2 mF = mj * cn2 * g * invr;            // m_j F (scaled)
-                const h2 cp = -mF * (pri2 + prj + Pi);
-                const h2 vc = mF * r2 * X;
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    accp[d] = __builtin_amdgcn_fdot2(cp, dr[d], accp[d], false);
-                    accv[d] = __builtin_amdgcn_fdot2(vc, dv[d], accv[d], false);
-                }
-                divv = __builtin_amdgcn_fdot2(mF, drdv, divv, false);
-            }
+    // Two pair interactions (the halves of record m), each kept only if its
+    // hit bit is set: the other half is then outside the window, beyond the
+    // support or the particle itself. Every lane runs this for every drain
+    // step; a lane with nothing left passes e = 0 (no hit bits), which adds
+    // exactly zero.
+    auto body = [&](uint32_t e) {
+        const uint32_t m = e & kRecMask;
+        const uint4 T = etab[e >> kMixRecBits];
+        const PosRec16 P = a.mx.pos[m];
+        const VelRec16 V = a.mx.vel[m];
+        const StateRec16 St = a.mx.st[m];
+        total += T.w;
+        const h2 dx = hfma(cxi2 - P.cx, Ls2, xi2 - P.x);
+        const h2 dy = (yi2 - P.y) - as_h2(T.x);
+        const h2 dz = (zi2 - P.z) - as_h2(T.y);
+        const h2 dvx = vxi - V.vx, dvy = vyi - V.vy, dvz = vzi - V.vz;
+        h2 r2 = dx * dx;
+        r2 = hfma(dy, dy, r2);
+        h2 drdv = dx * dvx;
+        drdv = hfma(dy, dvy, drdv);
+        if (D == 3) {
+            r2 = hfma(dz, dz, r2);
+            drdv = hfma(dz, dvz, drdv);
         }
-        total += qn;
-        qn = 0;
+        // keep r^2 in [1e-6, 16]: a masked half may be the particle itself
+        // (r = 0) or one far outside the support (r^2 may overflow FP16);
+        // the kernel weight is zero beyond r^2 = 4 either way
+        r2 = hmin(hmax(r2, tiny2), sixteen2);
+        if (VS == 1) drdv = as_h2(as_u32(drdv) & T.z);
+        const h2 mj = as_h2(as_u32(V.m) & T.z);
+        const h2 den = r2 + eps2;              // r^2 + eps h_i h_j
+        const h2 rho_s = rhoi + St.rho;
+        const h2 X = hrcp(den * rho_s);        // 1 / (den (rho_i + rho_j))
+        const h2 inv_den = X * rho_s, inv_rs = X * den;
+        const h2 mu = hmin(drdv, zero2) * inv_den;  // (h_i + h_j)/2 = 1
+        if (VS == 1) vsig2 = hmax(vsig2, -mu);
+        const h2 cbar = chi + St.ch;
+        // Pi (rho0 / c0^2) = (-2 alpha cbar + 2 beta mu) mu / (rho_i + rho_j)
+        const h2 Pi_ = (BT ? hfma(malpha2, cbar, beta2 * mu) : malpha2 * cbar) * mu * inv_rs;
+        const h2 invr = hrsq(r2);
+        const h2 rr = r2 * invr;               // r / h
+        const h2 aa = hmax(hfma(rr, hsplat(-0.5f), hsplat(1.f)), zero2);
+        const h2 bb = hmax(hfma(rr, hsplat(-0.5f), hsplat(0.5f)), zero2);
+        const h2 g = hfma(bb * bb, hsplat(12.f), (aa * aa) * hsplat(-3.f));
+        const h2 mF = mj * g * invr;           // m_j F (scaled)
+        const h2 cp = -(mF * (pri + St.pr + Pi_));
+        const h2 vc = mF * (r2 * X);
+        accp[0] = __builtin_amdgcn_fdot2(cp, dx, accp[0], false);
+        accp[1] = __builtin_amdgcn_fdot2(cp, dy, accp[1], false);
+        accv[0] = __builtin_amdgcn_fdot2(vc, dvx, accv[0], false);
+        accv[1] = __builtin_amdgcn_fdot2(vc, dvy, accv[1], false);
+        if (D == 3) {
+            accp[2] = __builtin_amdgcn_fdot2(cp, dz, accp[2], false);
+            accv[2] = __builtin_amdgcn_fdot2(vc, dvz, accv[2], false);
+        }
+        divv = __builtin_amdgcn_fdot2(mF, drdv, divv, false);
+    };
+    auto drain = [&]() {
+        const uint32_t k = min(qn, (uint32_t)K);
+        const uint32_t kmax = wave_max(k);
+        for (uint32_t t = 0; t < kmax; ++t) {
+            const uint32_t e = q[((qh + t) & (Q - 1)) * 64];
+            body(t < k ? e : 0u);
+        }
+        qh += k;
+        qn -= k;
     };
 
     auto row_bounds = [&](int r, uint32_t& b0, uint32_t& b1) {
@@ -217,43 +300,54 @@ __global__ __launch_bounds__(kBlock) void k_nforce_mixed(NFMixedArgs a) {
     row_bounds(0, nb0, nb1);
     uint32_t np0 = nb1 ? a.start[nb0] : 0u, npe = nb1 ? a.start[nb1] : 0u;
     for (int r = 0; r < NR; ++r) {
-        const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
         const uint32_t p0 = np0, len = npe - np0;
-        if (r + 1 < NR) {
+        if (r + 1 < NR) {  // prefetch the next row's range behind this row's scan
             row_bounds(r + 1, nb0, nb1);
             np0 = nb1 ? a.start[nb0] : 0u;
             npe = nb1 ? a.start[nb1] : 0u;
         }
+        const uint4 T = etab[r];
+        const h2 ry = yi2 - as_h2(T.x), rz = zi2 - as_h2(T.y);
+        // record k of the window holds particles 2 (m0 + k) and 2 (m0 + k) + 1;
+        // its low half lies in [p0, p0 + len) for k < npair except k = 0 when
+        // p0 is odd, its high half for k < hi_lim
         const uint32_t m0 = p0 >> 1;
-        const uint32_t npair = len ? ((p0 + len + 1) >> 1) - m0 : 0;
-        const h2 yi2 = hsplat(fmaf((float)(-dy), a.Ls, xs[1]));
-        const h2 zi2 = hsplat(D == 3 ? fmaf((float)(-dz), a.Ls, xs[2]) : 0.f);
-        constexpr int U = kScanUnroll / 2;  // records in flight (two candidates each)
-        for (uint32_t t = 0; __any(t < npair); t += U) {
-            ScanRec16 rec[U];
+        const uint32_t npair = len ? ((p0 + len + 1) >> 1) - m0 : 0u;
+        const uint32_t hi_lim = len ? (p0 + len - 2 * m0) >> 1 : 0u;
+        const bool even_start = (p0 & 1) == 0;
+        const uint32_t ebase = m0 | ((uint32_t)r << kMixRecBits);
+        const int32_t voff = (int32_t)(m0 * (uint32_t)sizeof(PosRec16));
+        const uint32_t npmax = wave_max(npair);
+        for (uint32_t t = 0; t < npmax; t += U) {
+            PosRec16 rec[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) rec[u] = a.mx.sc16[m0 + min(t + u, npair ? npair - 1 : 0u)];
+            for (int u = 0; u < U; ++u)
+                rec[u] = __builtin_bit_cast(
+                    PosRec16, __builtin_amdgcn_raw_buffer_load_b128(
+                                  pos_rsrc, voff + (int32_t)((t + u) * sizeof(PosRec16)), 0, 0));
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const h2 dx = (cxi2 - rec[u].cx) * Ls2 + xi2 - rec[u].x;
-                const h2 dyy = yi2 - rec[u].y;
-                h2 r2 = dx * dx + dyy * dyy;
-                if (D == 3) {
-                    const h2 dzz = zi2 - rec[u].z;
-                    r2 = dzz * dzz + r2;
-                }
-                const uint32_t j0 = 2 * (m0 + t + u), j1 = j0 + 1;
-                const bool ok = t + u < npair;
-                if (ok && j0 >= p0 && r2.x > (_Float16)0.f && r2.x < H2s) q[qn++] = j0 | ((uint32_t)r << kShift);
-                if (ok && j1 < p0 + len && r2.y > (_Float16)0.f && r2.y < H2s) q[qn++] = j1 | ((uint32_t)r << kShift);
+                const uint32_t k = t + u;
+                const h2 dx = hfma(cxi2 - rec[u].cx, Ls2, xi2) - rec[u].x;
+                const h2 dy = ry - rec[u].y;
+                const h2 dz = rz - rec[u].z;
+                h2 r2 = dx * dx;
+                r2 = hfma(dy, dy, r2);
+                if (D == 3) r2 = hfma(dz, dz, r2);
+                const bool lo = k < npair && (k != 0 || even_start) && r2.x < four;
+                const bool hi = k < hi_lim && r2.y < four;
+                // the tail slot is free (qn < Q here), so write it always and
+                // keep it only for a record with a hit
+                q[((qh + qn) & (Q - 1)) * 64] = (ebase + k) | (lo ? 1u << 30 : 0u) | (hi ? 1u << 31 : 0u);
+                qn += (lo || hi) ? 1u : 0u;
             }
-            if (__any(qn + 2 * U > (uint32_t)Q)) flush();
+            if (__any(qn > (uint32_t)(Q - U))) drain();
         }
     }
-    flush();
+    while (__any(qn > 0u)) drain();
     if (live) {
         const int64_t n = a.n;
-        const float4 qi4 = a.pk.pq[i];  // FP32 rho, P/rho^2, c, h
+        const float4 qi4 = a.pq[i];  // FP32 rho, P/rho^2, c, h
         const double c0 = a.c0, h = a.h;
         const double ks = c0 * c0 / h, kv = 4.0 * a.nu * c0 / (h * h);
 #pragma unroll
@@ -261,6 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_nforce_mixed(NFMixedArgs a) {
         a.divv[i] = -(c0 / h) * divv * (1.0 / ((double)qi4.x * (double)a.inv_rho0));
         const float vs = fmaxf((float)vsig2.x, (float)vsig2.y);
         a.dti[i] = a.c_cfl * h / ((double)qi4.z + (double)vs * c0);
+        --total;  // the particle itself passed the predicate (r = 0) and added zero
     }
     block_count_partial(total, 0u, a.P, blockIdx.x);
 }

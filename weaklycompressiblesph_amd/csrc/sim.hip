@@ -268,29 +268,23 @@ static void nforce_dispatch_h(wcsph_sim* s) {
     else nforce_dispatch_hv<D, UH, 0>(s);
 }
 
-template <int D, int R, int Q>
+template <int D, int R, int VS, int BT>
 static void launch_nforce_mixed(wcsph_sim* s) {
     NFMixedArgs a;
-    a.pk = s->pk;
     a.mx = s->mx;
+    a.px = s->pk.px;
+    a.pq = s->pk.pq;
+    a.pcell = s->pk.pcell;
     a.start = s->start;
     a.g = s->grid;
     a.n = s->n;
+    a.nrec = mixed_records(s->n);
     a.h = s->hmax;
-    a.inv_h = s->mc.inv_h;
-    a.Ls = (float)(s->grid.L / (double)s->hmax);
-    a.alpha = (float)s->p.alpha;
-    a.beta = (float)s->p.beta;
-    a.eps = (float)s->p.eps;
-    a.cnorm_s = (float)((D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI) / (D == 2 ? 8.0 : 16.0));
-    a.vsig_mode = s->p.vsig_mode;
-    auto splat = [](float v) { return h2{(_Float16)v, (_Float16)v}; };
-    a.k_eps = splat(a.eps);
-    a.k_cn = splat(a.cnorm_s);
-    a.k_malpha = splat(-2.f * a.alpha);
-    a.k_beta = splat(2.f * a.beta);
-    a.k_invh = splat(a.inv_h);
-    a.k_Ls = splat(a.Ls);
+    auto splat = [](double v) { return h2{(_Float16)(float)v, (_Float16)(float)v}; };
+    a.k_eps = splat(s->p.eps);
+    a.k_malpha = splat(-2.0 * s->p.alpha);
+    a.k_beta = splat(2.0 * s->p.beta);
+    a.k_Ls = splat(s->grid.L / (double)s->hmax);
     a.c0 = s->p.c0;
     a.nu = s->p.nu;
     a.c_cfl = s->p.c_cfl;
@@ -299,14 +293,20 @@ static void launch_nforce_mixed(wcsph_sim* s) {
     a.divv = s->divv;
     a.dti = s->dti;
     a.P = s->P;
-    hipLaunchKernelGGL((k_nforce_mixed<D, R, Q>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
+    hipLaunchKernelGGL((k_nforce_mixed<D, R, VS, BT>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc, 5);
+}
+
+template <int D, int BT>
+static void nforce_mixed_dispatch_b(wcsph_sim* s) {
+    if (s->p.vsig_mode) s->R == 1 ? launch_nforce_mixed<D, 1, 1, BT>(s) : launch_nforce_mixed<D, 2, 1, BT>(s);
+    else s->R == 1 ? launch_nforce_mixed<D, 1, 0, BT>(s) : launch_nforce_mixed<D, 2, 0, BT>(s);
 }
 
 template <int D>
 static void nforce_mixed_dispatch(wcsph_sim* s) {
-    if (s->R == 1) s->Q == 32 ? launch_nforce_mixed<D, 1, 32>(s) : launch_nforce_mixed<D, 1, 24>(s);
-    else s->Q == 32 ? launch_nforce_mixed<D, 2, 32>(s) : launch_nforce_mixed<D, 2, 24>(s);
+    if (s->p.beta != 0.0) nforce_mixed_dispatch_b<D, 1>(s);
+    else nforce_mixed_dispatch_b<D, 0>(s);
 }
 
 template <int D>
@@ -495,8 +495,13 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
             s->c64 = s->alloc<double>(n);
         }
         if (s->cfg.precision == WCSPH_PREC_MIXED) {
-            s->mx.sc16 = s->alloc<ScanRec16>((n + 1) / 2);
-            s->mx.q16 = s->alloc<Quad16>(n);
+            s->mx.pos = s->alloc<PosRec16>(mixed_records(n));
+            s->mx.vel = s->alloc<VelRec16>(mixed_records(n));
+            s->mx.st = s->alloc<StateRec16>(mixed_records(n));
+            // the high half of the last record stays zero when n is odd
+            WCSPH_HIP(hipMemsetAsync(s->mx.pos, 0, sizeof(PosRec16) * mixed_records(n), s->st));
+            WCSPH_HIP(hipMemsetAsync(s->mx.vel, 0, sizeof(VelRec16) * mixed_records(n), s->st));
+            WCSPH_HIP(hipMemsetAsync(s->mx.st, 0, sizeof(StateRec16) * mixed_records(n), s->st));
         }
         s->key = s->alloc<uint32_t>(n);
         s->slot = s->alloc<uint32_t>(n);
@@ -539,12 +544,16 @@ int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const d
             WCSPH_REQUIRE(s->uniform_h, WCSPH_EINVAL, "the mixed-precision path needs one smoothing length for all particles");
             WCSPH_REQUIRE(s->grid.nc[0] <= 2048, WCSPH_EINVAL,
                           "the mixed-precision path needs at most 2048 cells along x (FP16 cell index)");
+            WCSPH_REQUIRE(mixed_records(n) <= ((int64_t)1 << kMixRecBits), WCSPH_EINVAL,
+                          "the mixed-precision path holds at most 2^26 particles");
             const double hh = (double)s->hmax;
             s->mc.inv_h = (float)(1.0 / hh);
             s->mc.inv_c0 = (float)(1.0 / s->p.c0);
             s->mc.inv_rho0 = (float)(1.0 / s->p.rho0);
             s->mc.pr_scale = (float)(s->p.rho0 / (s->p.c0 * s->p.c0));
-            s->mc.m_scale = (float)(1.0 / (s->p.rho0 * std::pow(hh, D)));
+            const double cnorm = (D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI) / (D == 2 ? 8.0 : 16.0);
+            s->mc.m_scale = (float)(cnorm / (s->p.rho0 * std::pow(hh, D)));
+            s->mc.half_Ls = (float)(0.5 * s->grid.L / hh);
         }
         std::vector<uint8_t> t8(n);
         std::vector<uint32_t> id(n);

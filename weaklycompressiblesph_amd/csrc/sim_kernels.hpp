@@ -313,11 +313,11 @@ __global__ void k_reorder(const uint32_t* __restrict__ perm, const uint32_t* __r
         pr64[p] = pr;
         c64[p] = cs;
     }
-    extra(p, xl, c[0], rho, pr, cs, m);
+    extra(p, xl, c[0], vf, rho, pr, cs, m);
 }
 
 struct NoExtra {
-    __device__ void operator()(int64_t, const float*, int, double, double, double, double) const {}
+    __device__ void operator()(int64_t, const float*, int, const float*, double, double, double, double) const {}
 };
 
 // Walks the (2R+1)^D cells around particle i's cell (R = cells per
@@ -521,13 +521,21 @@ __device__ __forceinline__ float axis_gap(int d, float x, float Lf) {
     return fmaxf(fmaxf(lo - x, x - hi), 0.f);
 }
 
-// Wave-wide maximum, returned as a wave-uniform (scalar) value.
+// Wave-wide maximum of an unsigned value over a fully active wave, returned
+// as a wave-uniform (scalar) value: row shifts 1, 2, 4, 8 leave each 16-lane
+// row's maximum in its lane 15, row_bcast:15 / row_bcast:31 carry it across
+// rows, and lane 63 ends with the maximum of all 64 lanes.
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+    constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+    constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+    int x = (int)v;
+    x = max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(0, x, kRowShr1, 0xf, 0xf, false));
+    x = max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(0, x, kRowShr2, 0xf, 0xf, false));
+    x = max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(0, x, kRowShr4, 0xf, 0xf, false));
+    x = max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(0, x, kRowShr8, 0xf, 0xf, false));
+    x = max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(0, x, kRowBcast15, 0xa, 0xf, false));
+    x = max((unsigned)x, (unsigned)__builtin_amdgcn_update_dpp(0, x, kRowBcast31, 0xc, 0xf, false));
+    return (uint32_t)__builtin_amdgcn_readlane(x, 63);
 }
 
 constexpr int kScanUnroll = 4;  // candidate loads in flight per lane
