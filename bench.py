@@ -9,11 +9,15 @@ time step: kick-drift, cell-list rebuild by counting sort, neighbour pass,
 force pass, gravity/final kick and the global dt reduction, all in HBM.
 
 value = neighbour pair interactions evaluated in the timed steps / wall time
-of those steps. The roofline entry is for the force kernel (k_force), from
-HIP events recorded on the simulation's stream around every timed launch.
-The CPU baseline times the FP64 oracle's force pass (the reference CPU
-functor, src/force.h:31-60, over cell-list candidates, OpenMP) on the same
-developed state, on the host's cores.
+of those steps, for the mixed-precision path (packed FP16 pair arithmetic,
+FP32 accumulation, FP64 integration); the FP32 plain path runs the same
+steps beside it ("paths", "mixed_over_fp32"). The roofline entry is for the
+headline force kernel, from HIP events recorded on the simulation's stream
+around every timed launch. The CPU baseline times the FP64 oracle's force
+pass (the reference CPU functor, src/force.h:31-60, over cell-list
+candidates, OpenMP) on the same developed state, on the host's cores. The
+serving entry runs a seeded Poisson stream of 16 dam-break requests through
+the continuous-batching loop (p50/p99 completion latency).
 """
 import argparse
 import json
@@ -25,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = {3: 73, 2: 63}       # algorithmic flops per pair interaction
-PEAK_FP32_TFLOPS = 157.3             # MI355X published FP32 peak
+PEAK_VALU_TFLOPS = 157.3             # MI355X published FP32 (packed) / packed-FP16 vector peak
 PEAK_HBM_GBS = 8000.0
 
 
@@ -117,13 +121,13 @@ def accuracy_study(W, dim, dx, marks, chunk, early=100):
     return out
 
 
-def serving_study(W, seed, count, rate, max_slots, chunk, precision, budget_gib):
-    """Continuous-batching run of a seeded Poisson request stream against the
-    same stream served one request at a time (max_slots = 1)."""
+def serving_study(W, seed, count, rate, max_slots, chunk, precision, budget_gib, compare=True):
+    """Continuous-batching run of a seeded Poisson request stream, and (with
+    compare) the same stream served one request at a time (max_slots = 1)."""
     import numpy as np
     reqs = W.poisson_workload(seed, count, rate, precision=precision)
     runs, results = {}, []
-    for slots in (max_slots, 1):
+    for slots in ((max_slots, 1) if compare else (max_slots,)):
         t0 = time.perf_counter()
         res = W.serve(reqs, hbm_budget_bytes=int(budget_gib * (1 << 30)), max_slots=slots, chunk_steps=chunk)
         wall = time.perf_counter() - t0
@@ -141,7 +145,7 @@ def serving_study(W, seed, count, rate, max_slots, chunk, precision, budget_gib)
         results.append(res)
         print(f"# slots {slots}: makespan {makespan:.2f} s", file=sys.stderr, flush=True)
     keys = ("steps", "pairs", "sim_time", "front", "column", "kinetic")
-    identical = all(a[k] == b[k] for a, b in zip(*results) for k in keys)
+    identical = all(a[k] == b[k] for a, b in zip(*results) for k in keys) if compare else None
     return {"workload": f"{count} dam-break requests, Poisson arrivals at {rate}/s (seed {seed}), 2D/3D, "
                         f"20k-2M particles, 500-5000 steps, {precision}",
             "requests": [{"dim": r.geom.dim, "dx": r.geom.dx, "steps": r.steps, "arrival_s": r.arrival_s}
@@ -155,7 +159,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "mixed", "fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed", "fp64"],
+                    help="headline path (the mixed and FP32 paths are both run unless --only)")
+    ap.add_argument("--only", action="store_true", help="run the headline path alone")
     ap.add_argument("--dim", type=int, default=3, choices=[2, 3])
     ap.add_argument("--dx", type=float, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 skips it)")
@@ -167,7 +173,7 @@ def main():
     ap.add_argument("--t-end", type=float, default=1.0)
     ap.add_argument("--serve", action="store_true",
                     help="run a seeded request stream through the serving loop (p50/p99 latency, makespan)")
-    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving run (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-seed", type=int, default=2026)
@@ -182,9 +188,8 @@ def main():
     dim = args.dim
     dx = args.dx or (0.0126 if dim == 3 else 0.02)
     if args.serve:
-        prec = args.precision if args.precision != "fp32" or "--precision" in sys.argv else "mixed"
         print(json.dumps(serving_study(W, args.serve_seed, args.serve_requests, args.serve_rate, args.serve_slots,
-                                       50, prec, 64.0)), flush=True)
+                                       50, args.precision, 64.0)), flush=True)
         return
     if args.accuracy:
         marks = [m for m in (0.25, 0.5, 1.0) if m <= args.t_end] or [args.t_end]
@@ -194,17 +199,85 @@ def main():
     desc = W.dam_break_desc(dim, dx)
     st = W.dam_break_state(desc, params)
     n = len(st["dens"])
-    print(f"# {dim}D dam break dx={dx}: {st['n_fluid']} fluid + {st['n_wall']} wall = {n} particles, "
-          f"precision {args.precision}", file=sys.stderr, flush=True)
-    sim = W.Simulation.from_state(params, st, precision=args.precision, cells_per_support=args.cells_per_support,
+    print(f"# {dim}D dam break dx={dx}: {st['n_fluid']} fluid + {st['n_wall']} wall = {n} particles",
+          file=sys.stderr, flush=True)
+    # the headline path first, then the others it is compared with
+    order = [args.precision] + [p for p in ("mixed", "fp32") if p != args.precision and not args.only]
+    runs, keep = {}, None
+    for prec in order:
+        sim, r = run_path(W, params, st, prec, args)
+        runs[prec] = r
+        if keep is None:
+            keep = sim
+        else:
+            sim.close()
+    head = runs[args.precision]
+    K = args.steps
+    dtype = {"fp32": "fp32", "mixed": "fp16x2/fp32acc", "fp64": "fp64"}[args.precision]
+    force_ms = head["force_ms_per_launch"]
+    achieved = FLOP_PER_PAIR[dim] * head["pairs_per_step"] / (force_ms * 1e-3) / 1e12
+    kname = {"fp32": "k_nforce", "mixed": "k_nforce_mixed", "fp64": "k_force64"}[args.precision]
+    line = {
+        "metric": "pair-interactions/s",
+        "value": head["value"],
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (dam-break lattice initial condition, developed by the warmup steps)",
+        "config": {"workload": f"{dim}D dam break, dx={dx} m, {n} particles ({st['n_fluid']} fluid), "
+                               f"{args.precision} force path, full device-resident steps",
+                   "particles": n, "dx": dx, "precision": args.precision,
+                   "cells_per_support": args.cells_per_support or 2,
+                   "step_window": f"steps {args.warmup + 1}-{args.warmup + K}"},
+        "paths": runs,
+        "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved, "peak": PEAK_VALU_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_VALU_TFLOPS, "traffic": None,
+                     "flop_per_pair": FLOP_PER_PAIR[dim], "avg_launch_ms": force_ms,
+                     "note": "algorithmic flops of the pair body over the force launch time (HIP events on the "
+                             "simulation stream); the peak is MI355X's published FP32/packed-FP16 vector rate"},
+    }
+    if "mixed" in runs and "fp32" in runs:
+        line["mixed_over_fp32"] = {
+            "step_pairs_per_s": runs["mixed"]["value"] / runs["fp32"]["value"],
+            "force_kernel_pairs_per_s": runs["mixed"]["force_kernel_pairs_per_s"] /
+                                        runs["fp32"]["force_kernel_pairs_per_s"]}
+    if dim == 3 and abs(dx - 0.0126) < 1e-12:
+        prof = profiled_traffic(kname)
+        if prof:
+            line["roofline"]["traffic_profiled"] = prof
+    line["hbm_copy_GBps"] = W.copy_probe_gbps(1 << 30, 10)
+    if rank == 0 and args.cpu_seconds > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(f"# cpu baseline on {threads} threads", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(keep, params, args.cpu_seconds, threads)
+    keep.close()
+    if rank == 0 and args.serve_requests > 0 and not args.only:
+        print(f"# serving {args.serve_requests} requests", file=sys.stderr, flush=True)
+        sv = serving_study(W, args.serve_seed, args.serve_requests, args.serve_rate, args.serve_slots, 50,
+                           "mixed", 64.0, compare=False)
+        line["serving"] = {"workload": sv["workload"], "max_slots": sv["max_slots"],
+                           **sv["runs"][f"slots_{args.serve_slots}"]}
+    if rank == 0:
+        print(json.dumps(line), flush=True)
+
+
+def run_path(W, params, st, prec, args):
+    """Warm a simulation up, then time args.steps full steps (HIP events on
+    the simulation stream per stage). Returns the simulation and its figures."""
+    sim = W.Simulation.from_state(params, st, precision=prec, cells_per_support=args.cells_per_support,
                                   queue_depth=args.queue_depth)
     sim.initial_state = st
     sim.init()
     t0 = time.perf_counter()
     sim.step(args.warmup)
     sim.sync()
-    print(f"# warmup {args.warmup} steps: {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
-
+    print(f"# {prec}: warmup {args.warmup} steps {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     s0 = sim.stats()
     sim.set_timing(True)
     sim.sync()
@@ -215,51 +288,40 @@ def main():
     s1 = sim.stats()
     tm = sim.timing()
     sim.set_timing(False)
+    K = args.steps
     wall = t1 - t0
     pairs = s1["pairs_total"] - s0["pairs_total"]
-    K = args.steps
     force_ms = tm["force"] / K
-    pairs_launch = pairs / K
-    achieved = FLOP_PER_PAIR[dim] * pairs_launch / (force_ms * 1e-3) / 1e12
-    dtype = {"fp32": "fp32", "mixed": "fp16x2/fp32", "fp64": "fp64"}[args.precision]
-    line = {
-        "metric": "pair-interactions/s",
+    return sim, {
         "value": pairs / wall,
-        "unit": "pair-interactions/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / K,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": dtype,
-        "data": "synthetic (dam-break lattice initial condition, developed by the warmup steps)",
-        "config": {"workload": f"{dim}D dam break, dx={dx} m, {n} particles ({st['n_fluid']} fluid), "
-                               f"{args.precision} force path, full device-resident steps",
-                   "particles": n, "dx": dx, "precision": args.precision,
-                   "cells_per_support": args.cells_per_support or 2, "queue_depth": args.queue_depth or 24,
-                   "step_window":
-                       f"steps {args.warmup + 1}-{args.warmup + K}"},
         "steps_per_s": K / wall,
-        "pairs_per_step": pairs_launch,
-        "force_kernel_pairs_per_s": pairs_launch / (force_ms * 1e-3),
+        "pairs_per_step": pairs / K,
+        "force_ms_per_launch": force_ms,
+        "force_kernel_pairs_per_s": pairs / K / (force_ms * 1e-3),
         "stage_ms_per_step": {k: tm[k] / K for k in ("kick_bin_reorder", "neighbour", "force", "final", "step")},
         "sim_time_s": s1["time"],
         "escaped": s1["escaped"],
         "overflow": s1["overflow"],
-        "roofline": {"bound": "valu", "kernel": "k_force", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "flop_per_pair": FLOP_PER_PAIR[dim], "avg_launch_ms": force_ms},
     }
-    line["hbm_copy_GBps"] = W.copy_probe_gbps(1 << 30, 10)
-    if rank == 0 and args.cpu_seconds > 0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        print(f"# cpu baseline on {threads} threads", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(sim, params, args.cpu_seconds, threads)
-    sim.close()
-    if rank == 0:
-        print(json.dumps(line), flush=True)
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of the force kernel on this workload, from the
+    committed rocprofv3 summary: FETCH_SIZE x 2 + WRITE_SIZE (KiB per
+    dispatch; the factors are those of a 1 GiB copy kernel read the same way)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_dam_break_3d_1.07M.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 4 and f[0].startswith(kernel + "<") and f[-4] in ("FETCH_SIZE", "WRITE_SIZE"):
+            vals[f[-4]] = float(f[-2])
+    if len(vals) < 2:
+        return None
+    b = (2.0 * vals["FETCH_SIZE"] + vals["WRITE_SIZE"]) * 1024.0
+    return {"bytes_per_launch": b, "source": os.path.relpath(path, ROOT)}
 
 
 if __name__ == "__main__":
