@@ -124,6 +124,7 @@ struct NFMixedArgs {
     double* divv;
     double* dti;
     Partials P;
+    Scalars* s;
 };
 
 // Tuning constants (build-time overridable): ring queue entries per lane
@@ -274,46 +275,47 @@ void k_nforce_mixed(NFMixedArgs a) {
         qn -= k;
     };
 
+    // Row r of the stencil (offsets dy, dz): the index range [b0, b1) of
+    // the cells of that row inside lane i's support chord; b0 = b1 = 0 when
+    // the row is outside the grid or the support. The window is widened by
+    // the 1e-4 margin of hc, which covers the approximate square root.
+    const uint32_t ncx = (uint32_t)a.g.nc[0], ncy = (uint32_t)a.g.nc[1], ncz = (uint32_t)a.g.nc[2];
+    const uint32_t own = (uint32_t)((c[2] * (int)ncy + c[1]) * (int)ncx + c[0]);  // own cell index
+    const float xi0L = xi[0] * invLf;
+    const int xa_min = -c[0], xb_max = (int)ncx - 1 - c[0];
     auto row_bounds = [&](int r, uint32_t& b0, uint32_t& b1) {
         const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
-        const int cy = c[1] + dy, cz = c[2] + dz;
-        b0 = b1 = 0;
-        if (live && cy >= 0 && cy < a.g.nc[1] && (D == 2 || (cz >= 0 && cz < a.g.nc[2]))) {
-            const float gy = axis_gap(dy, xi[1], Lf);
-            const float gz = D == 3 ? axis_gap(dz, xi[2], Lf) : 0.f;
-            const float g2 = gy * gy + gz * gz;
-            if (g2 < hc2) {
-                const float w = sqrtf(hc2 - g2);
-                int xa = (int)floorf((xi[0] - w) * invLf), xb = (int)floorf((xi[0] + w) * invLf);
-                xa = max(max(xa, -R), -c[0]);
-                xb = min(min(xb, R), a.g.nc[0] - 1 - c[0]);
-                if (xa <= xb) {
-                    const uint32_t row = D == 3 ? (uint32_t)((cz * a.g.nc[1] + cy) * a.g.nc[0])
-                                                : (uint32_t)(cy * a.g.nc[0]);
-                    b0 = row + c[0] + xa;
-                    b1 = row + c[0] + xb + 1;
-                }
-            }
-        }
+        const int srow = (dz * (int)ncy + dy) * (int)ncx;  // wave-uniform
+        const float gy = axis_gap(dy, xi[1], Lf);
+        const float gz = D == 3 ? axis_gap(dz, xi[2], Lf) : 0.f;
+        const float g2 = fmaf(gz, gz, gy * gy);
+        const float w = __builtin_amdgcn_sqrtf(fmaxf(hc2 - g2, 0.f)) * invLf;
+        const int xa = max(max((int)floorf(xi0L - w), -R), xa_min);
+        const int xb = min(min((int)floorf(xi0L + w), R), xb_max);
+        const bool ok = live && (uint32_t)(c[1] + dy) < ncy && (D == 2 || (uint32_t)(c[2] + dz) < ncz) &&
+                        g2 < hc2 && xa <= xb;
+        b0 = ok ? own + srow + xa : 0u;
+        b1 = ok ? own + srow + xb + 1 : 0u;
     };
     uint32_t nb0, nb1;
     row_bounds(0, nb0, nb1);
-    uint32_t np0 = nb1 ? a.start[nb0] : 0u, npe = nb1 ? a.start[nb1] : 0u;
+    uint32_t np0 = a.start[nb0], npe = a.start[nb1];
     for (int r = 0; r < NR; ++r) {
-        const uint32_t p0 = np0, len = npe - np0;
+        const uint32_t p0 = np0, pend = npe;
         if (r + 1 < NR) {  // prefetch the next row's range behind this row's scan
             row_bounds(r + 1, nb0, nb1);
-            np0 = nb1 ? a.start[nb0] : 0u;
-            npe = nb1 ? a.start[nb1] : 0u;
+            np0 = a.start[nb0];
+            npe = a.start[nb1];
         }
         const uint4 T = etab[r];
         const h2 ry = yi2 - as_h2(T.x), rz = zi2 - as_h2(T.y);
         // record k of the window holds particles 2 (m0 + k) and 2 (m0 + k) + 1;
-        // its low half lies in [p0, p0 + len) for k < npair except k = 0 when
-        // p0 is odd, its high half for k < hi_lim
+        // its low half lies in [p0, pend) for k < npair except k = 0 when p0
+        // is odd, its high half for k < hi_lim (an empty window starting at
+        // an odd index scans one record with neither half valid)
         const uint32_t m0 = p0 >> 1;
-        const uint32_t npair = len ? ((p0 + len + 1) >> 1) - m0 : 0u;
-        const uint32_t hi_lim = len ? (p0 + len - 2 * m0) >> 1 : 0u;
+        const uint32_t npair = ((pend + 1) >> 1) - m0;
+        const uint32_t hi_lim = (pend >> 1) - m0;
         const bool even_start = (p0 & 1) == 0;
         const uint32_t ebase = m0 | ((uint32_t)r << kMixRecBits);
         const int32_t voff = (int32_t)(m0 * (uint32_t)sizeof(PosRec16));
@@ -357,7 +359,7 @@ void k_nforce_mixed(NFMixedArgs a) {
         a.dti[i] = a.c_cfl * h / ((double)qi4.z + (double)vs * c0);
         --total;  // the particle itself passed the predicate (r = 0) and added zero
     }
-    block_count_partial(total, 0u, a.P, blockIdx.x);
+    block_count_atomic(total, a.s, true);
 }
 
 }  // namespace wcsph

@@ -3,7 +3,8 @@
 // One step (src/main.cpp:89-133 without the Shepard filter) is queued as:
 //   k_scalars(commit dt)      getTimeStepGlobal result of the previous step
 //   k_kick_drift              initialKick + fullDrift + predict (FP64)
-//   k_fill/k_bin/scan/scatter/k_cell_sort   cell list by counting sort
+//   k_bin/k_scan_reduce/k_scan_apply/k_scatter/k_cell_sort   cell list by
+//                             counting sort
 //   k_reorder                 permute the state into cell order + EoS,
 //                             compact FP32 force records (cell-local x)
 //   k_nforce                  fused neighbour search + hydro force (FP32;
@@ -11,7 +12,11 @@
 //                             dispatch/retrieve); the FP64 path runs
 //                             k_neigh (lists) + k_force64 instead
 //   k_final                   gravity + finalKick + density floor + dt bound
-// Nothing returns to the host between steps; dt lives on the device.
+// The fused force passes add their per-block pair counts into the step
+// scalars with one integer atomic per block (order-independent); k_final's
+// per-block dt bounds are reduced in index order by the next step's
+// k_scalars. Nothing returns to the host between steps; dt lives on the
+// device.
 #include <cmath>
 #include <cstring>
 
@@ -140,13 +145,14 @@ static void build_grid(wcsph_sim* s, const double* pos, const double* h) {
     s->count = s->alloc<uint32_t>(ncell);
     s->start = s->alloc<uint32_t>(ncell + 1);
     s->part = s->alloc<uint32_t>(s->npart);
+    // the scan clears count after reading it, ready for the next pass
+    WCSPH_HIP(hipMemsetAsync(s->count, 0, sizeof(uint32_t) * ncell, s->st));
 }
 
 static void bin_and_reorder(wcsph_sim* s) {
     const int64_t n = s->n;
     Master& src = s->m[s->cur];
     Master& dst = s->m[1 - s->cur];
-    hipLaunchKernelGGL(k_fill_u32, dim3(s->nb(s->ncell)), dim3(kBlock), 0, s->st, s->count, s->ncell, 0u);
     if (s->D == 3)
         hipLaunchKernelGGL(k_bin<3>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src.pos, n, s->grid, s->key, s->slot,
                            s->count, s->sc);
@@ -154,9 +160,7 @@ static void bin_and_reorder(wcsph_sim* s) {
         hipLaunchKernelGGL(k_bin<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src.pos, n, s->grid, s->key, s->slot,
                            s->count, s->sc);
     hipLaunchKernelGGL(k_scan_reduce, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, s->st, s->part, s->npart);
-    hipLaunchKernelGGL(k_scan_final, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part,
-                       s->start);
+    hipLaunchKernelGGL(k_scan_apply, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part, s->start);
     hipLaunchKernelGGL(k_scatter, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->key, s->slot, s->start, s->perm, n);
     hipLaunchKernelGGL(k_cell_sort, dim3(s->nb(s->ncell)), dim3(kBlock), 0, s->st, s->start, s->perm, s->ncell);
     const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
@@ -252,8 +256,8 @@ static void launch_nforce(wcsph_sim* s) {
     a.divv = s->divv;
     a.dti = s->dti;
     a.P = s->P;
+    a.s = s->sc;
     hipLaunchKernelGGL((k_nforce<D, R, Q, UH, VS>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc, 5);
 }
 
 template <int D, bool UH, int VS>
@@ -293,8 +297,8 @@ static void launch_nforce_mixed(wcsph_sim* s) {
     a.divv = s->divv;
     a.dti = s->dti;
     a.P = s->P;
+    a.s = s->sc;
     hipLaunchKernelGGL((k_nforce_mixed<D, R, VS, BT>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc, 5);
 }
 
 template <int D, int BT>
@@ -345,7 +349,6 @@ static void final_stage(wcsph_sim* s, int kick) {
     else
         hipLaunchKernelGGL(k_final<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->dti, s->p.g,
                            s->p.nu, s->p.rho_floor, kick, s->sc, s->P, s->n);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc, 2);
 }
 
 static hipEvent_t* step_events(wcsph_sim* s) {
@@ -361,7 +364,7 @@ static hipEvent_t* step_events(wcsph_sim* s) {
 static void one_step(wcsph_sim* s) {
     hipEvent_t* ev = s->timing ? step_events(s) : nullptr;
     if (ev) WCSPH_HIP(hipEventRecord(ev[0], s->st));
-    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, s->st, s->sc, 1);
+    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 1);
     const Master& m = s->m[s->cur];
     if (s->D == 3)
         hipLaunchKernelGGL(k_kick_drift<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->divv, s->sc,
@@ -578,7 +581,7 @@ int wcsph_sim_init(wcsph_sim* s) {
     return guarded([&] {
         WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "upload a state first");
         WCSPH_HIP(hipSetDevice(s->device));
-        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, s->st, s->sc, 0);
+        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 0);
         bin_and_reorder(s);
         neigh_force(s, nullptr, nullptr, nullptr);
         final_stage(s, 0);
@@ -643,7 +646,7 @@ int wcsph_sim_force(wcsph_sim* s) {
     return guarded([&] {
         WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "upload a state first");
         WCSPH_HIP(hipSetDevice(s->device));
-        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, s->st, s->sc, 2);
+        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
         WCSPH_HIP(hipEventRecord(s->ev[0], s->st));
         bin_and_reorder(s);
         neigh_force(s, s->ev[1], s->ev[2], s->ev[3]);
@@ -758,7 +761,7 @@ int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t 
         WCSPH_REQUIRE(!lists || cap > 0, WCSPH_EINVAL, "cap must be positive with lists");
         if (s->cfg.precision != WCSPH_PREC_FP64) {
             // the fused pass keeps no lists: rebuild them for the current cell order
-            hipLaunchKernelGGL(k_scalars, dim3(1), dim3(64), 0, s->st, s->sc, 2);
+            hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
             neigh_lists(s, 0);
             WCSPH_HIP(hipGetLastError());
         }

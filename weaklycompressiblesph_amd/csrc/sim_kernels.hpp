@@ -85,15 +85,23 @@ __device__ __forceinline__ uint32_t cell_key(const Grid& g, const int* c) {
     return D == 3 ? (uint32_t)((c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0]) : (uint32_t)(c[1] * g.nc[0] + c[0]);
 }
 
-__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-// mode 0: full reset; 1: commit dt = dtmin, time += dt, step += 1;
-// 2: reset the per-pass counters only. Modes 0 and 1 also reset dtmin.
-__global__ void k_scalars(Scalars* s, int mode) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// mode 0: full reset; 1: commit dt = the minimum of the last k_final's
+// per-block bounds (reduced here in index order), time += dt, step += 1;
+// 2: reset the per-pass counters only. One block of kBlock threads.
+__global__ __launch_bounds__(kBlock) void k_scalars(Scalars* s, const double* __restrict__ dtmin, int64_t nblk,
+                                                     int mode) {
+    __shared__ double sh[kBlock];
+    if (mode == 1) {
+        double d = 1.0e30;
+        for (int64_t b = threadIdx.x; b < nblk; b += kBlock) d = dtmin[b] < d ? dtmin[b] : d;
+        sh[threadIdx.x] = d;
+        __syncthreads();
+        for (int off = kBlock / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) sh[threadIdx.x] = sh[threadIdx.x + off] < sh[threadIdx.x] ? sh[threadIdx.x + off] : sh[threadIdx.x];
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x != 0) return;
     if (mode == 0) {
         s->dt = 0.0;
         s->time = 0.0;
@@ -102,11 +110,11 @@ __global__ void k_scalars(Scalars* s, int mode) {
         s->overflow = 0;
     }
     if (mode == 1) {
-        s->dt = __longlong_as_double((long long)s->dtmin_bits);
+        s->dt = sh[0];
+        s->dtmin_bits = (unsigned long long)__double_as_longlong(sh[0]);
         s->time += s->dt;
         s->step += 1;
     }
-    if (mode != 2) s->dtmin_bits = (unsigned long long)__double_as_longlong(1.0e30);
     s->pairs_last = 0;
     s->escaped = 0;
     s->maxcnt = 0;
@@ -132,26 +140,44 @@ __global__ void k_kick_drift(Master m, const double* __restrict__ acc, const dou
     }
 }
 
-// Counting sort, pass 1: cell key and slot within the cell.
+// Counting sort, pass 1: cell key and slot within the cell. The state is in
+// the previous pass's cell order, so neighbouring lanes mostly share a key:
+// each run of equal keys within a wave takes its slots with one atomic, made
+// by the run's first lane and broadcast to the rest.
 template <int D>
 __global__ void k_bin(const double* __restrict__ pos, int64_t n, Grid g, uint32_t* __restrict__ key,
                       uint32_t* __restrict__ slot, uint32_t* __restrict__ count, Scalars* s) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double x[D];
+    const bool live = i < n;
+    uint32_t k = 0xffffffffu;
+    if (live) {
+        double x[D];
 #pragma unroll
-    for (int d = 0; d < D; ++d) x[d] = pos[d * n + i];
-    int c[3] = {0, 0, 0};
-    bool esc = false;
-    cell_of<D>(g, x, c, esc);
-    const uint32_t k = cell_key<D>(g, c);
-    key[i] = k;
-    slot[i] = atomicAdd(&count[k], 1u);
-    if (esc) atomicAdd(&s->escaped, 1u);
+        for (int d = 0; d < D; ++d) x[d] = pos[d * n + i];
+        int c[3] = {0, 0, 0};
+        bool esc = false;
+        cell_of<D>(g, x, c, esc);
+        k = cell_key<D>(g, c);
+        key[i] = k;
+        if (esc) atomicAdd(&s->escaped, 1u);
+    }
+    const int lane = threadIdx.x & 63;
+    const uint32_t kp = (uint32_t)__shfl_up((int)k, 1, 64);
+    const bool lead = lane == 0 || kp != k;  // first lane of a run (dead lanes form their own run)
+    const unsigned long long leads = __ballot(lead);
+    const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;
+    const int first = 63 - __clzll(leads & upto);                    // first lane of my run
+    const unsigned long long after = leads & ~upto;
+    const int next = after ? __ffsll((long long)after) - 1 : 64;      // first lane of the next run
+    uint32_t base = 0;
+    if (lead && live) base = atomicAdd(&count[k], (uint32_t)(next - lane));
+    base = (uint32_t)__shfl((int)base, first, 64);
+    if (live) slot[i] = base + (uint32_t)(lane - first);
 }
 
-// Exclusive scan of count[0..m) into start[0..m], start[m] = total.
-// 3 phases over tiles of 1024 elements (256 threads x 4).
+// Exclusive scan of count[0..m) into start[0..m], start[m] = total, over
+// tiles of 1024 elements (256 threads x 4): k_scan_reduce writes each tile's
+// total, k_scan_apply finishes each tile.
 constexpr int kScanTile = 1024;
 
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
@@ -182,37 +208,35 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restri
     if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_partials(uint32_t* part, int64_t np) {
-    __shared__ uint32_t sh[kBlock];
-    uint32_t carry = 0;
-    for (int64_t b = 0; b < np; b += kBlock) {
-        const int64_t i = b + threadIdx.x;
-        const uint32_t v = i < np ? part[i] : 0u;
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan(v, sh, total);
-        if (i < np) part[i] = carry + ex;
-        carry += total;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_scan_final(const uint32_t* __restrict__ in, int64_t m,
+// Second scan phase: each tile adds up the totals of the tiles before it
+// (every tile re-reads at most a few thousand words, all in L2), scans its
+// own counts, writes start and clears count for the next binning pass.
+__global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ count, int64_t m,
                                                         const uint32_t* __restrict__ part,
-                                                        uint32_t* __restrict__ out) {
+                                                        uint32_t* __restrict__ start) {
     __shared__ uint32_t sh[kBlock];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
+    const int64_t tile = blockIdx.x;
+    uint32_t pre = 0;
+    for (int64_t t = threadIdx.x; t < tile; t += kBlock) pre += part[t];
+    uint32_t before;
+    block_exclusive_scan(pre, sh, before);
+    const int64_t base = tile * kScanTile + threadIdx.x * 4;
     uint32_t v[4], sum = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        v[k] = (base + k < m) ? in[base + k] : 0u;
+        v[k] = (base + k < m) ? count[base + k] : 0u;
         sum += v[k];
     }
     uint32_t total;
-    uint32_t run = part[blockIdx.x] + block_exclusive_scan(sum, sh, total);
+    uint32_t run = before + block_exclusive_scan(sum, sh, total);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (base + k < m) out[base + k] = run;
+        if (base + k < m) {
+            start[base + k] = run;
+            count[base + k] = 0u;
+        }
         run += v[k];
-        if (base + k == m - 1) out[m] = run;
+        if (base + k == m - 1) start[m] = run;
     }
 }
 
@@ -418,11 +442,15 @@ __device__ __forceinline__ void block_count_partial(uint32_t cnt, uint32_t over,
 
 // Reduce the per-block partials in index order into the step scalars.
 // what: bit 0 pair counts (added to pairs_last, and to pairs_total if
-// bit 2), bit 1 the dt bound (into dtmin_bits).
-__global__ __launch_bounds__(kBlock) void k_reduce_partials(Partials P, Scalars* s, int what) {
-    __shared__ unsigned long long sh_p[kBlock];
-    __shared__ uint32_t sh_m[kBlock], sh_o[kBlock];
-    __shared__ double sh_d[kBlock];
+// bit 2), bit 1 the dt bound (into dtmin_bits). One block; sh is LDS
+// scratch (a caller may lend memory it no longer uses).
+struct ReduceScratch {
+    unsigned long long p[kBlock];
+    double d[kBlock];
+    uint32_t m[kBlock], o[kBlock];
+};
+
+__device__ __forceinline__ void reduce_partials_block(const Partials& P, Scalars* s, int what, ReduceScratch& sh) {
     unsigned long long p = 0;
     uint32_t m = 0, o = 0;
     double d = 1.0e30;
@@ -434,28 +462,66 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(Partials P, Scalars*
         }
         if (what & 2) d = P.dtmin[b] < d ? P.dtmin[b] : d;
     }
-    sh_p[threadIdx.x] = p;
-    sh_m[threadIdx.x] = m;
-    sh_o[threadIdx.x] = o;
-    sh_d[threadIdx.x] = d;
+    const int t = threadIdx.x;
+    sh.p[t] = p;
+    sh.m[t] = m;
+    sh.o[t] = o;
+    sh.d[t] = d;
     __syncthreads();
     for (int off = kBlock / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) {
-            sh_p[threadIdx.x] += sh_p[threadIdx.x + off];
-            sh_m[threadIdx.x] = sh_m[threadIdx.x + off] > sh_m[threadIdx.x] ? sh_m[threadIdx.x + off] : sh_m[threadIdx.x];
-            sh_o[threadIdx.x] |= sh_o[threadIdx.x + off];
-            sh_d[threadIdx.x] = sh_d[threadIdx.x + off] < sh_d[threadIdx.x] ? sh_d[threadIdx.x + off] : sh_d[threadIdx.x];
+        if (t < off) {
+            sh.p[t] += sh.p[t + off];
+            sh.m[t] = sh.m[t + off] > sh.m[t] ? sh.m[t + off] : sh.m[t];
+            sh.o[t] |= sh.o[t + off];
+            sh.d[t] = sh.d[t + off] < sh.d[t] ? sh.d[t + off] : sh.d[t];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         if (what & 1) {
-            s->pairs_last += sh_p[0];
-            if (what & 4) s->pairs_total += sh_p[0];
-            s->maxcnt = sh_m[0] > s->maxcnt ? sh_m[0] : s->maxcnt;
-            s->overflow |= sh_o[0];
+            s->pairs_last += sh.p[0];
+            if (what & 4) s->pairs_total += sh.p[0];
+            s->maxcnt = sh.m[0] > s->maxcnt ? sh.m[0] : s->maxcnt;
+            s->overflow |= sh.o[0];
         }
-        if (what & 2) s->dtmin_bits = (unsigned long long)__double_as_longlong(sh_d[0]);
+        if (what & 2) s->dtmin_bits = (unsigned long long)__double_as_longlong(sh.d[0]);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(Partials P, Scalars* s, int what) {
+    __shared__ ReduceScratch sh;
+    reduce_partials_block(P, s, what, sh);
+}
+
+// Block totals of the fused force passes, added straight into the step
+// scalars with one integer atomic each per block: integer sums, maxima and
+// ors do not depend on the order the blocks arrive in.
+__device__ __forceinline__ void block_count_atomic(uint32_t cnt, Scalars* s, bool to_total) {
+    __shared__ unsigned long long sh_sum[kBlock / 64];
+    __shared__ uint32_t sh_max[kBlock / 64];
+    unsigned long long v = cnt;
+    uint32_t mx = cnt;
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_down(v, off, 64);
+        const uint32_t o = __shfl_down(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh_sum[wv] = v;
+        sh_max[wv] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        uint32_t m = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            t += sh_sum[k];
+            m = sh_max[k] > m ? sh_max[k] : m;
+        }
+        atomicAdd(&s->pairs_last, t);
+        if (to_total) atomicAdd(&s->pairs_total, t);
+        atomicMax(&s->maxcnt, m);
     }
 }
 
@@ -514,6 +580,7 @@ struct NFArgs {
     double* divv;
     double* dti;
     Partials P;
+    Scalars* s;
 };
 
 __device__ __forceinline__ float axis_gap(int d, float x, float Lf) {
@@ -693,7 +760,7 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
         a.divv[i] = (double)divv / (double)I.rho;
         a.dti[i] = a.c_cfl * (double)I.h / ((double)I.c + (double)vsig);
     }
-    block_count_partial(total, 0u, a.P, blockIdx.x);
+    block_count_atomic(total, a.s, true);
 }
 
 // FP64 force pass over neighbour lists (the GPU oracle path): pair
