@@ -154,6 +154,102 @@ def serving_study(W, seed, count, rate, max_slots, chunk, precision, budget_gib,
             "results_identical_batched_vs_one_at_a_time": identical}
 
 
+def fdps_study(W, dim, dx, warmup, group_edge=5.0):
+    """The FDPS drop-in (GPU_WC::HydrDispatchKernel / HydrRetrieveKernel)
+    end to end on a developed state: host records in, host results out,
+    PCIe and host packing included. FDPS itself is not here, so the walks
+    are formed FDPS-style: particles grouped by cubes of group_edge * dx
+    (about 128 per group in the fluid, FDPS's Ngroup), each group's j list
+    all particles within its bounding box grown by 2 h_max (FDPS's r_search
+    of src/class.h:123-125)."""
+    import numpy as np
+    params = W.dam_break_params(dim)
+    st = W.dam_break_state(W.dam_break_desc(dim, dx), params)
+    with W.Simulation.from_state(params, st, precision="fp32") as sim:
+        sim.init()
+        sim.step(warmup)
+        sim.force()
+        out = sim.download()
+        pairs = sim.stats()["pairs_last"]
+    n = out["dens"].shape[0]
+    h = st["h"][0]
+    rho = out["dens"]
+    B = params.c0 ** 2 * params.rho0 / params.gamma
+    r = rho / params.rho0
+    pres = np.maximum(B * (r ** params.gamma - 1.0), 0.0)
+    snds = np.sqrt(np.maximum(B * params.gamma / params.rho0 * r ** (params.gamma - 1.0), 1e-16))
+    epi_all = np.zeros(n, W.epi_dtype(dim))
+    epj_all = np.zeros(n, W.epj_dtype(dim))
+    for a in (epi_all, epj_all):
+        a["pos"], a["vel"], a["smth"], a["dens"], a["pres"], a["snds"], a["visc"] = \
+            out["pos"].T, out["vel"].T, h, rho, pres, snds, params.nu
+    epj_all["mass"] = st["mass"]
+    t0 = time.perf_counter()
+    x = out["pos"].T
+    G = group_edge * dx
+    lo = x.min(axis=0)
+    c = np.floor((x - lo) / G).astype(np.int64)
+    nc = c.max(axis=0) + 1
+    key = c[:, 0]
+    for d in range(1, dim):
+        key = key * nc[d] + c[:, d]
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    bounds = np.flatnonzero(np.diff(ks)) + 1
+    groups = np.split(order, bounds)
+    cell_of = {int(k): g for k, g in zip(ks[np.r_[0, bounds]], groups)}
+    halo = 2.0 * h * (1.0 + 1e-6)
+    epi, epj = [], []
+    offs = [np.array(o) for o in np.ndindex(*([3] * dim))]
+    for g in groups:
+        cg = c[g[0]]
+        cand = []
+        for o in offs:
+            cc = cg + o - 1
+            if np.any(cc < 0) or np.any(cc >= nc):
+                continue
+            kk = cc[0]
+            for d in range(1, dim):
+                kk = kk * nc[d] + cc[d]
+            m = cell_of.get(int(kk))
+            if m is not None:
+                cand.append(m)
+        cand = np.concatenate(cand)
+        blo, bhi = x[g].min(axis=0) - halo, x[g].max(axis=0) + halo
+        inside = np.all((x[cand] >= blo) & (x[cand] <= bhi), axis=1)
+        epi.append(epi_all[g])
+        epj.append(epj_all[cand[inside]])
+    t_build = time.perf_counter() - t0
+    nw = len(epi)
+    ni = [len(e) for e in epi]
+    nj = [len(e) for e in epj]
+    res = [np.zeros(k, W.res_dtype(dim)) for k in ni]
+    W.init(params)
+    limit = 16384  # FDPS's n_walk_limit (src/kernel.cuh:4)
+    runs = []
+    for rep in range(3):
+        t0 = time.perf_counter()
+        kernel_ms = 0.0
+        for b in range(0, nw, limit):
+            e = min(nw, b + limit)
+            W.check(W.HydrDispatchKernel(0, e - b, epi[b:e], ni[b:e], epj[b:e], nj[b:e]))
+            W.check(W.HydrRetrieveKernel(0, e - b, ni[b:e], res[b:e]))
+            kernel_ms += W.last_kernel_ms(0)
+        runs.append((time.perf_counter() - t0, kernel_ms))
+    W.finalize()
+    wall, kms = min(runs)
+    cand_pairs = sum(a * b for a, b in zip(ni, nj))
+    return {"workload": f"{dim}D dam break dx={dx}, {n} particles after {warmup} steps, FDPS-style walks "
+                        f"({nw} groups of <= {max(ni)} i, {int(np.mean(nj))} j on average)",
+            "pairs": pairs, "candidate_pairs": cand_pairs,
+            "pairs_per_s_end_to_end": pairs / wall, "end_to_end_ms": 1e3 * wall,
+            "kernel_ms": kms, "pairs_per_s_kernel": pairs / (kms * 1e-3),
+            "host_records_MB": (sum(ni) * epi_all.itemsize + sum(nj) * epj_all.itemsize) / 1e6,
+            "walk_build_s": t_build,
+            "note": "end to end = dispatch + retrieve wall time, best of 3: host packing of the AoS "
+                    "records, H2D, kernel, D2H and unpacking"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +267,8 @@ def main():
     ap.add_argument("--accuracy", action="store_true",
                     help="run the FP32 and mixed paths against the FP64 path to --t-end and print their errors")
     ap.add_argument("--t-end", type=float, default=1.0)
+    ap.add_argument("--fdps", action="store_true",
+                    help="time the FDPS drop-in end to end (host records, PCIe included) on a developed state")
     ap.add_argument("--serve", action="store_true",
                     help="run a seeded request stream through the serving loop (p50/p99 latency, makespan)")
     ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving run (0 skips it)")
@@ -187,6 +285,9 @@ def main():
 
     dim = args.dim
     dx = args.dx or (0.0126 if dim == 3 else 0.02)
+    if args.fdps:
+        print(json.dumps(fdps_study(W, dim, dx, args.warmup)), flush=True)
+        return
     if args.serve:
         print(json.dumps(serving_study(W, args.serve_seed, args.serve_requests, args.serve_rate, args.serve_slots,
                                        50, args.precision, 64.0)), flush=True)
