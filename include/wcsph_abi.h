@@ -63,7 +63,9 @@ typedef struct {
     double nu;          /* kinematic viscosity for the simulation (src/EoS.h:38) */
     double rho_floor;   /* density floor after each step (src/init/Dam2.h:111; 0 = off) */
     int32_t device;     /* HIP device ordinal */
-    int32_t reserved_;
+    int32_t shepard_interval; /* device-resident step: Shepard density filter (src/force.h:10-30)
+                                 at init and on steps with step % interval == 0
+                                 (src/main.cpp:75,105; src/param.h:12 uses 30); 0 = off */
 } wcsph_params;
 
 /* Fill p with the reference CPU functor's constants (src/force.h:32,45-47,

@@ -36,7 +36,8 @@ class SimParams(C.Structure):
     _fields_ = [("dim", C.c_int), ("vsig_mode", C.c_int), ("alpha", C.c_double),
                 ("beta", C.c_double), ("eps", C.c_double), ("c_cfl", C.c_double),
                 ("rho0", C.c_double), ("c0", C.c_double), ("gamma", C.c_double),
-                ("g", C.c_double), ("nu", C.c_double), ("rho_floor", C.c_double)]
+                ("g", C.c_double), ("nu", C.c_double), ("rho_floor", C.c_double),
+                ("shepard_interval", C.c_int)]
 
 
 class _CState(C.Structure):
@@ -76,6 +77,7 @@ def lib():
         L.orc_make_walks.restype = C.c_int64
         L.orc_make_walks.argtypes = [C.c_int, C.c_int64, f64p, C.c_double, C.c_int,
                                      C.POINTER(C.c_int64), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_shepard.argtypes = [C.c_int, C.c_int64, f64p, f64p, f64p, f64p, f64p, C.c_int]
         L.orc_ic_dam_break.restype = C.c_int64
         L.orc_ic_dam_break.argtypes = [C.c_int] + [C.c_double] * 6 + [C.c_int] + [C.c_double] * 5 + \
                                       [C.POINTER(_CState)]
@@ -197,10 +199,19 @@ def make_walks(pos, r_search, group_max=128):
 # whole-step oracle
 # ----------------------------------------------------------------------------
 def sim_params(dim, *, alpha=0.1, beta=0.0, eps=0.01, c_cfl=0.25, vsig_mode=0, rho0=1000.0,
-               c0=None, col_h=2.0, gamma=7.0, g=9.81, nu=1.0e-6, rho_floor=0.0):
+               c0=None, col_h=2.0, gamma=7.0, g=9.81, nu=1.0e-6, rho_floor=0.0, shepard_interval=0):
     if c0 is None:
         c0 = 10.0 * np.sqrt(2.0 * g * col_h)
-    return SimParams(dim, vsig_mode, alpha, beta, eps, c_cfl, rho0, c0, gamma, g, nu, rho_floor)
+    return SimParams(dim, vsig_mode, alpha, beta, eps, c_cfl, rho0, c0, gamma, g, nu, rho_floor, shepard_interval)
+
+
+def shepard(pos, h, mass, dens, nthreads=1):
+    """Shepard density filter (src/force.h:10-30); pos (dim, n) FP64."""
+    dim, n = pos.shape
+    out = np.zeros(n)
+    lib().orc_shepard(dim, n, np.ascontiguousarray(pos, np.float64), np.ascontiguousarray(h, np.float64),
+                      np.ascontiguousarray(mass, np.float64), np.ascontiguousarray(dens, np.float64), out, nthreads)
+    return out
 
 
 class State:

@@ -510,6 +510,75 @@ int64_t orc_sim_force(const orc_sim_params* p, orc_state* s, int nthreads) {
     return pairs;
 }
 
+void orc_shepard(int dim, int64_t n, const double* x, const double* h, const double* mass,
+                 const double* dens, double* out, int nthreads) {
+    /* src/force.h:10-30 with the Gather tree of src/main.cpp:65 (j within 2 h_i) */
+    double hmax = 0, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int64_t i = 0; i < n; ++i) if (h[i] > hmax) hmax = h[i];
+    for (int k = 0; k < dim; ++k) {
+        lo[k] = hi[k] = x[k * n];
+        for (int64_t i = 1; i < n; ++i) {
+            if (x[k * n + i] < lo[k]) lo[k] = x[k * n + i];
+            if (x[k * n + i] > hi[k]) hi[k] = x[k * n + i];
+        }
+    }
+    const double L = 2.0 * hmax * 1.0001;
+    int64_t nc[3] = {1, 1, 1};
+    for (int k = 0; k < dim; ++k) nc[k] = (int64_t)((hi[k] - lo[k]) / L) + 1;
+    const int64_t ncell = nc[0] * nc[1] * nc[2];
+    int64_t* cstart = (int64_t*)calloc((size_t)ncell + 1, sizeof(int64_t));
+    int64_t* cidx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    int64_t* ccell = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t c[3] = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) c[k] = (int64_t)((x[k * n + i] - lo[k]) / L);
+        ccell[i] = (c[2] * nc[1] + c[1]) * nc[0] + c[0];
+        cstart[ccell[i] + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; ++c) cstart[c + 1] += cstart[c];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)ncell);
+    memcpy(fill, cstart, sizeof(int64_t) * (size_t)ncell);
+    for (int64_t i = 0; i < n; ++i) cidx[fill[ccell[i]]++] = i;
+    free(fill);
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t c[3] = {0, 0, 0};
+        for (int k = 0; k < dim; ++k) c[k] = (int64_t)((x[k * n + i] - lo[k]) / L);
+        double one = 0.0, sum = 0.0;
+        const int64_t zr = dim == 3 ? 1 : 0;
+        for (int64_t dz = -zr; dz <= zr; ++dz)
+            for (int64_t dy = -1; dy <= 1; ++dy)
+                for (int64_t dx = -1; dx <= 1; ++dx) {
+                    const int64_t cx = c[0] + dx, cy = c[1] + dy, cz = c[2] + dz;
+                    if (cx < 0 || cy < 0 || cz < 0 || cx >= nc[0] || cy >= nc[1] || cz >= nc[2]) continue;
+                    const int64_t cc = (cz * nc[1] + cy) * nc[0] + cx;
+                    for (int64_t t = cstart[cc]; t < cstart[cc + 1]; ++t) {
+                        const int64_t j = cidx[t];
+                        double r2 = 0.0;
+                        for (int k = 0; k < dim; ++k) {
+                            const double d = x[k * n + i] - x[k * n + j];
+                            r2 += d * d;
+                        }
+                        const double W = orc_kernel_W(dim, sqrt(r2), h[i]);  /* src/force.h:17,23 */
+                        one += W * mass[j] / dens[j];                         /* src/force.h:18 */
+                        sum += mass[j] * W;
+                    }
+                }
+        out[i] = sum / one;                                                  /* src/force.h:24 */
+    }
+    free(cstart); free(cidx); free(ccell);
+}
+
+static void sim_shepard(const orc_sim_params* p, orc_state* s, int nthreads) {
+    double* out = (double*)malloc(sizeof(double) * (size_t)(s->n > 0 ? s->n : 1));
+    orc_shepard(p->dim, s->n, s->x, s->h, s->mass, s->dens, out, nthreads);
+    memcpy(s->dens, out, sizeof(double) * (size_t)s->n);   /* copyFromForce, src/class.h:116-118 */
+    free(out);
+}
+
 void orc_sim_eos(const orc_sim_params* p, orc_state* s) {
     /* CalcPressure, src/force.h:3-9 */
     for (int64_t i = 0; i < s->n; ++i) {
@@ -531,9 +600,10 @@ static void sim_gravity_post(const orc_sim_params* p, orc_state* s, int final_ki
 }
 
 void orc_sim_init(const orc_sim_params* p, orc_state* s, int nthreads) {
-    /* src/main.cpp:69-79 (without the Shepard filter) */
+    /* src/main.cpp:69-79: pressure, force, filter, gravity */
     orc_sim_eos(p, s);
     orc_sim_force(p, s, nthreads);
+    if (p->shepard_interval > 0) sim_shepard(p, s, nthreads);
     sim_gravity_post(p, s, 0);
 }
 
@@ -573,6 +643,7 @@ void orc_sim_step(const orc_sim_params* p, orc_state* s, int nthreads) {
         }
     }
     s->time += dt;
+    if (p->shepard_interval > 0 && s->step % p->shepard_interval == 0) sim_shepard(p, s, nthreads); /* :105 */
     orc_sim_eos(p, s);
     orc_sim_force(p, s, nthreads);
     sim_gravity_post(p, s, 1);

@@ -99,6 +99,7 @@ typedef struct {
     double g;          /* gravity along the last axis, src/init/Dam2.h:104-108 */
     double nu;         /* kinematic viscosity, src/EoS.h:38 */
     double rho_floor;  /* src/init/Dam2.h:111 (0 disables) */
+    int shepard_interval; /* steps between Shepard filters, src/param.h:12 (0 disables) */
 } orc_sim_params;
 
 typedef struct {
@@ -118,9 +119,15 @@ int64_t orc_ic_dam_break(int dim, double dx, double col_w, double col_h,
 
 /* CalcPressure (src/force.h:3-9) over the state */
 void orc_sim_eos(const orc_sim_params* p, orc_state* s);
-/* CalcPressure, first force + gravity (src/main.cpp:69-79) */
+/* ApplyFilter (src/force.h:10-30): Shepard density re-initialisation,
+ * out_i = sum_j m_j W(r_ij, h_i) / sum_j (m_j / rho_j) W(r_ij, h_i), both sums
+ * over every j (i itself included) within i's support 2 h_i. x is SoA [dim][n]. */
+void orc_shepard(int dim, int64_t n, const double* x, const double* h, const double* mass,
+                 const double* dens, double* out, int nthreads);
+/* CalcPressure, first force, the filter (if enabled) + gravity (src/main.cpp:69-79) */
 void orc_sim_init(const orc_sim_params* p, orc_state* s, int nthreads);
-/* one step of src/main.cpp:89-133 (without the Shepard filter) */
+/* one step of src/main.cpp:89-133; the filter on steps with
+ * step % shepard_interval == 0, after the drift (src/main.cpp:105) */
 void orc_sim_step(const orc_sim_params* p, orc_state* s, int nthreads);
 /* getTimeStepGlobal (src/integral.h:2-12) */
 double orc_global_dt(const orc_sim_params* p, const orc_state* s);

@@ -429,3 +429,126 @@ def test_cell_local_predicate(O, dim):
     assert np.array_equal(got, got.T)
     cnt2 = O.neighbours_brute_cells(pos, h, lo - 0.37 * L, L, nc + 1)
     assert np.abs(cnt2 - cnt).sum() <= max(2, int(border.sum()))
+
+
+# ---------------------------------------------------------------------------
+# Shepard filter (src/force.h:10-30)
+# ---------------------------------------------------------------------------
+def test_shepard_two_particles_closed_form(O):
+    # rho_i' = (m_i W(0,h_i) + m_j W(r,h_i)) / (m_i/rho_i W(0,h_i) + m_j/rho_j W(r,h_i))
+    for dim in (2, 3):
+        r, hi, hj = 0.37, 0.3, 0.45
+        pos = np.zeros((dim, 2)); pos[0, 1] = r
+        h = np.array([hi, hj]); m = np.array([1.3, 0.7]); rho = np.array([1010.0, 990.0])
+        out = O.shepard(pos, h, m, rho)
+        for i, j in ((0, 1), (1, 0)):
+            w0, wr = O.kernel_W(dim, 0.0, h[i]), O.kernel_W(dim, r, h[i])
+            want = (m[i] * w0 + m[j] * wr) / (m[i] / rho[i] * w0 + m[j] / rho[j] * wr)
+            assert out[i] == pytest.approx(want, rel=1e-14)
+        # the kernel is evaluated at h_i, not the mean: j beyond 2 h_i is ignored by i
+        pos[0, 1] = 2.0 * hi * 1.01
+        out = O.shepard(pos, h, m, rho)
+        assert out[0] == pytest.approx(rho[0], rel=1e-14)
+        assert out[1] != pytest.approx(rho[1], rel=1e-6)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shepard_linear_field_on_lattice(O, dim):
+    # on a uniform lattice with volumes V = dx^D and m_j = rho_j V, the filter is
+    # the W-weighted mean of rho_j; W is symmetric on the lattice, so a linear
+    # field comes back exactly in the interior (a wrong volume weighting, e.g.
+    # m_j instead of m_j / rho_j in the denominator, breaks this)
+    dx, h = 0.1, 0.13
+    g = np.arange(10 if dim == 3 else 16) * dx
+    grids = np.meshgrid(*([g] * dim), indexing="ij")
+    pos = np.vstack([a.ravel() for a in grids])
+    n = pos.shape[1]
+    rho = 1000.0 + 3.0 * pos[0] - 2.0 * pos[1]
+    m = rho * dx ** dim
+    out = O.shepard(pos, np.full(n, h), m, rho, 4)
+    lo, hi = g[0] + 2 * h, g[-1] - 2 * h
+    interior = np.all((pos > lo) & (pos < hi), axis=0)
+    assert interior.sum() > 0
+    assert np.allclose(out[interior], rho[interior], rtol=1e-13)
+
+
+def test_shepard_uniform_density_is_fixed_point(O):
+    # with one density everywhere the filter returns it exactly, for any positions
+    rng = np.random.default_rng(7)
+    for dim in (2, 3):
+        n = 400
+        pos = rng.uniform(0, 1, (dim, n))
+        h = rng.uniform(0.08, 0.12, n)
+        m = rng.uniform(0.5, 2.0, n)
+        out = O.shepard(pos, h, m, np.full(n, 1000.0))
+        assert np.allclose(out, 1000.0, rtol=1e-13)
+
+
+def test_shepard_smooths_noise_brute_force(O):
+    # against a direct O(n^2) restatement: noisy density on a jittered lattice
+    rng = np.random.default_rng(3)
+    g = np.arange(12) * 0.1
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    pos = np.vstack([X.ravel(), Y.ravel()]) + rng.uniform(-0.01, 0.01, (2, g.size ** 2))
+    n = pos.shape[1]
+    h = np.full(n, 0.13); m = np.full(n, 10.0)
+    rho = 1000.0 * (1 + 0.01 * rng.standard_normal(n))
+    out = O.shepard(pos, h, m, rho, 4)
+    ref = np.empty(n)
+    for i in range(n):
+        r = np.linalg.norm(pos - pos[:, i:i + 1], axis=0)
+        W = np.array([O.kernel_W(2, rr, h[i]) for rr in r])
+        ref[i] = np.sum(m * W) / np.sum(m / rho * W)
+    assert np.allclose(out, ref, rtol=1e-13)
+    interior = np.all((pos > 0.3) & (pos < 0.8), axis=0)
+    assert np.std(out[interior]) < 0.5 * np.std(rho[interior])
+
+
+def test_init_applies_shepard(O):
+    # src/main.cpp:69-79: the first filter runs after the first force pass
+    p = O.sim_params(2, shepard_interval=30)
+    q = O.sim_params(2, shepard_interval=0)
+    a = O.State.dam_break(2, 0.05, col_w=0.5, col_h=0.6, tank_l=1.0, tank_h=1.0)
+    b = O.State.dam_break(2, 0.05, col_w=0.5, col_h=0.6, tank_l=1.0, tank_h=1.0)
+    a.init(p)
+    b.init(q)
+    assert np.allclose(a.dens, O.shepard(b.x, b.h, b.mass, b.dens), rtol=1e-14)
+    assert np.array_equal(a.a, b.a)  # the force pass saw the unfiltered density
+
+
+def test_step_composition_with_shepard(O):
+    # a filtering step composed by hand: kick-drift, the filter on the drifted
+    # state, pressure from the filtered density, force, gravity, final kick
+    s = O.State.dam_break(2, 0.1, col_w=0.5, col_h=0.6, tank_l=1.0, tank_h=1.0)
+    p = O.sim_params(2, alpha=0.3, shepard_interval=2)
+    s.init(p)
+    s.step(p)
+    s.step(p)
+    assert s.step_count == 2  # 2 % 2 == 0: the next step filters
+    x0, v0, a0, rho0, dv0 = s.x.copy(), s.v.copy(), s.a.copy(), s.dens.copy(), s.divv.copy()
+    dt = s.global_dt(p)
+    s.step(p)
+    fl = s.type == 0
+    vh = np.where(fl, v0 + 0.5 * dt * a0, 0.0)
+    x1 = np.where(fl, x0 + dt * vh, x0)
+    vp = np.where(fl, v0 + dt * a0, v0)
+    rho1 = O.shepard(x1, s.h, s.mass, rho0 - rho0 * dt * dv0)
+    n = s.n
+    epi = np.zeros(n, O.epi_dtype(2))
+    epj = np.zeros(n, O.epj_dtype(2))
+    P = np.array([O.eos_pressure(r, p.rho0, p.c0, p.gamma) for r in rho1])
+    c = np.array([O.eos_soundspeed(r, p.rho0, p.c0, p.gamma) for r in rho1])
+    for a in (epi, epj):
+        a["pos"], a["vel"], a["smth"], a["dens"], a["pres"], a["snds"], a["visc"] = x1.T, vp.T, s.h, rho1, P, c, p.nu
+    epj["mass"] = s.mass
+    ref = O.hydro_force(O.force_params(2, alpha=p.alpha, c_cfl=p.c_cfl), epi, epj)
+    acc = ref["acc"].T.copy()
+    acc[1] -= p.g
+    assert np.allclose(s.dens, rho1, rtol=1e-14)
+    assert np.allclose(s.a, acc, rtol=1e-10, atol=1e-10 * np.abs(acc).max())
+    # and the following step (index 3) does not filter
+    rho2 = s.dens.copy()
+    dv2 = s.divv.copy()
+    dt2 = s.global_dt(p)
+    s.step(p)
+    assert np.allclose(s.dens, rho2 - rho2 * dt2 * dv2, rtol=1e-14)

@@ -36,6 +36,7 @@ int wcsph_params_reference(int32_t dim, wcsph_params* p) {
     p->nu = 1.0e-6;     // src/EoS.h:38
     p->rho_floor = 100.0;  // src/init/Dam2.h:111
     p->device = 0;
+    p->shepard_interval = 30;  // src/param.h:12
     return WCSPH_OK;
 }
 

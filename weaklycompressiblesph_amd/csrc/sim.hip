@@ -58,6 +58,8 @@ struct wcsph_sim {
     bool uploaded = false, forced = false;
     bool timing = false;
     bool use_graphs = true;
+    int64_t hstep = 0;         // steps queued since init (the Shepard filter schedule)
+    double* rho_tmp = nullptr;  // filtered densities before they are committed
     hipGraphExec_t graph2[2] = {nullptr, nullptr};  // two steps starting from cur = 0 / 1
     std::vector<hipEvent_t> tev;  // 5 events per timed step, pooled
     int64_t tsteps = 0;
@@ -90,6 +92,7 @@ void check_params(const wcsph_params* p) {
     WCSPH_REQUIRE(p->dim == 2 || p->dim == 3, WCSPH_EINVAL, "dim must be 2 or 3");
     WCSPH_REQUIRE(p->rho0 > 0 && p->c0 > 0 && p->gamma > 0 && p->c_cfl > 0 && p->eps >= 0, WCSPH_EINVAL,
                   "rho0, c0, gamma, c_cfl must be positive and eps non-negative");
+    WCSPH_REQUIRE(p->shepard_interval >= 0, WCSPH_EINVAL, "shepard_interval must be non-negative");
 }
 
 int64_t sim_bytes(int D, int64_t n, int K, int64_t ncell_guess) {
@@ -361,7 +364,48 @@ static hipEvent_t* step_events(wcsph_sim* s) {
     return &s->tev[(size_t)s->tsteps * 5];
 }
 
-static void one_step(wcsph_sim* s) {
+// Shepard filter over the current cell order (records must be up to date).
+static void shepard_pass(wcsph_sim* s) {
+    const Master& m = s->m[s->cur];
+    const int64_t n = s->n;
+    const double cnorm = s->D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI;  // src/kernel.h:35-41
+    const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
+    EosConsts e{s->p.rho0, B, s->p.gamma};
+    const bool mixed = s->cfg.precision == WCSPH_PREC_MIXED;
+    if (s->D == 3) {
+        if (s->R == 1)
+            hipLaunchKernelGGL((k_shepard<3, 1>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
+                               n, cnorm, s->rho_tmp);
+        else
+            hipLaunchKernelGGL((k_shepard<3, 2>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
+                               n, cnorm, s->rho_tmp);
+        if (mixed)
+            hipLaunchKernelGGL((k_apply_density<3, MixedExtra<3>>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk,
+                               s->rho_tmp, s->pr64, s->c64, e, MixedExtra<3>{s->mx, s->mc}, n);
+        else
+            hipLaunchKernelGGL((k_apply_density<3, NoExtra>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk,
+                               s->rho_tmp, s->pr64, s->c64, e, NoExtra{}, n);
+    } else {
+        if (s->R == 1)
+            hipLaunchKernelGGL((k_shepard<2, 1>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
+                               n, cnorm, s->rho_tmp);
+        else
+            hipLaunchKernelGGL((k_shepard<2, 2>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
+                               n, cnorm, s->rho_tmp);
+        if (mixed)
+            hipLaunchKernelGGL((k_apply_density<2, MixedExtra<2>>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk,
+                               s->rho_tmp, s->pr64, s->c64, e, MixedExtra<2>{s->mx, s->mc}, n);
+        else
+            hipLaunchKernelGGL((k_apply_density<2, NoExtra>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk,
+                               s->rho_tmp, s->pr64, s->c64, e, NoExtra{}, n);
+    }
+}
+
+static bool shepard_due(const wcsph_sim* s, int64_t step) {
+    return s->p.shepard_interval > 0 && step % s->p.shepard_interval == 0;
+}
+
+static void one_step(wcsph_sim* s, bool filter) {
     hipEvent_t* ev = s->timing ? step_events(s) : nullptr;
     if (ev) WCSPH_HIP(hipEventRecord(ev[0], s->st));
     hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 1);
@@ -373,6 +417,7 @@ static void one_step(wcsph_sim* s) {
         hipLaunchKernelGGL(k_kick_drift<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->divv, s->sc,
                            s->n);
     bin_and_reorder(s);
+    if (filter) shepard_pass(s);  // after the drift, before the pressure (src/main.cpp:105-106)
     if (ev) neigh_force(s, ev[1], ev[2], ev[3]);
     else neigh_force(s, nullptr, nullptr, nullptr);
     final_stage(s, 1);
@@ -487,6 +532,7 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
             m.id = s->alloc<uint32_t>(n);
         }
         s->acc = s->alloc<double>(D * n);
+        s->rho_tmp = s->alloc<double>(n);
         s->divv = s->alloc<double>(n);
         s->dti = s->alloc<double>(n);
         s->pk.px = s->alloc<float4>(n);
@@ -584,7 +630,9 @@ int wcsph_sim_init(wcsph_sim* s) {
         hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 0);
         bin_and_reorder(s);
         neigh_force(s, nullptr, nullptr, nullptr);
+        if (s->p.shepard_interval > 0) shepard_pass(s);  // after the first force (src/main.cpp:75)
         final_stage(s, 0);
+        s->hstep = 0;
         WCSPH_HIP(hipGetLastError());
     });
 }
@@ -598,8 +646,8 @@ static hipGraphExec_t two_step_graph(wcsph_sim* s) {
     const int cur0 = s->cur;
     hipGraph_t g = nullptr;
     WCSPH_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-    one_step(s);
-    one_step(s);
+    one_step(s, false);
+    one_step(s, false);
     WCSPH_HIP(hipStreamEndCapture(s->st, &g));
     WCSPH_REQUIRE(s->cur == cur0, WCSPH_ESTATE, "two-step capture did not return to its parity");
     const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -613,11 +661,19 @@ int wcsph_sim_step(wcsph_sim* s, int32_t nsteps) {
         WCSPH_REQUIRE(s && s->uploaded && s->forced, WCSPH_ESTATE, "call wcsph_sim_init first");
         WCSPH_REQUIRE(nsteps >= 0, WCSPH_EINVAL, "nsteps < 0");
         WCSPH_HIP(hipSetDevice(s->device));
-        int k = 0;
-        if (s->use_graphs && !s->timing) {
-            for (; k + 2 <= nsteps; k += 2) WCSPH_HIP(hipGraphLaunch(two_step_graph(s), s->st));
+        // pairs of steps without a filter replay the captured graph
+        for (int k = 0; k < nsteps;) {
+            if (s->use_graphs && !s->timing && k + 2 <= nsteps && !shepard_due(s, s->hstep) &&
+                !shepard_due(s, s->hstep + 1)) {
+                WCSPH_HIP(hipGraphLaunch(two_step_graph(s), s->st));
+                s->hstep += 2;
+                k += 2;
+            } else {
+                one_step(s, shepard_due(s, s->hstep));
+                ++s->hstep;
+                ++k;
+            }
         }
-        for (; k < nsteps; ++k) one_step(s);
         WCSPH_HIP(hipGetLastError());
     });
 }

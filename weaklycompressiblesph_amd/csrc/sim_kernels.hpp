@@ -269,6 +269,18 @@ struct EosConsts {
     double rho0, B, gamma;
 };
 
+// Murnaghan/Tait (src/EoS.h:78-86, CalcPressure src/force.h:3-9): P/rho^2
+// and the sound speed of density rho, in FP64.
+__device__ __forceinline__ void tait(const EosConsts& e, double rho, double& pr, double& cs) {
+    const double r = rho / e.rho0;
+    double P = e.B * (pow(r, e.gamma) - 1.0);
+    P = P > 0.0 ? P : 0.0;
+    double c2 = e.B * e.gamma / e.rho0 * pow(r, e.gamma - 1.0);
+    c2 = c2 > 1.0e-16 ? c2 : 1.0e-16;
+    pr = P / (rho * rho);
+    cs = sqrt(c2);
+}
+
 // Gather the master state into cell order and derive the force inputs
 // (CalcPressure, src/force.h:3-9 with Murnaghan, src/EoS.h:78-86).
 // Cell-local coordinate along one axis: x - (lo + c L), both steps in FP64
@@ -323,12 +335,8 @@ __global__ void k_reorder(const uint32_t* __restrict__ perm, const uint32_t* __r
     dst.h[p] = h;
     dst.type[p] = src.type[o];
     dst.id[p] = src.id[o];
-    const double r = rho / e.rho0;
-    double P = e.B * (pow(r, e.gamma) - 1.0);
-    P = P > 0.0 ? P : 0.0;
-    double c2 = e.B * e.gamma / e.rho0 * pow(r, e.gamma - 1.0);
-    c2 = c2 > 1.0e-16 ? c2 : 1.0e-16;
-    const double pr = P / (rho * rho), cs = sqrt(c2);
+    double pr, cs;
+    tait(e, rho, pr, cs);
     pk.px[p] = make_float4(xl[0], xl[1], xl[2], (float)c[0]);
     pk.pv[p] = make_float4(vf[0], vf[1], vf[2], (float)m);
     pk.pq[p] = make_float4((float)rho, (float)pr, (float)cs, (float)h);
@@ -761,6 +769,80 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
         a.dti[i] = a.c_cfl * (double)I.h / ((double)I.c + (double)vsig);
     }
     block_count_atomic(total, a.s, true);
+}
+
+// Shepard density filter (ApplyFilter, src/force.h:10-30, with the Gather
+// tree of src/main.cpp:65): for every i,
+//   rho_i <- sum_j m_j W(r_ij, h_i) / sum_j (m_j / rho_j) W(r_ij, h_i)
+// over all j within 2 h_i, i itself included, in FP64 from the master state
+// (cell order), gathered over the cell stencil. The result goes to out; it is
+// committed by k_apply_density once every particle has read the old values.
+template <int D, int R>
+__global__ __launch_bounds__(kBlock) void k_shepard(Master m, Packed pk, const uint32_t* __restrict__ start, Grid g,
+                                                    int64_t n, double cnorm, double* __restrict__ out) {
+    const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = m.pos[d * n + i];
+    const double H = 2.0 * m.h[i], H2 = H * H, invH = 1.0 / H;
+    double norm = cnorm;  // C_D / H^D (src/kernel.h:42-48)
+#pragma unroll
+    for (int d = 0; d < D; ++d) norm *= invH;
+    const float4 xf = pk.px[i];
+    const float xl[3] = {xf.x, xf.y, xf.z};
+    int c[3];
+    key_cell<D>(g, pk.pcell[i], c);
+    const float hc = (float)H * 1.0001f;
+    Stencil<D, R> st;
+    st.k = -1;
+    st.advance(g, start, c, xl, hc * hc);
+    double one = 0.0, sum = 0.0;
+    while (st.k < Stencil<D, R>::NS) {
+        for (uint32_t j = st.p; j < st.pend; ++j) {
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const double dd = xi[d] - m.pos[d * n + j];
+                r2 += dd * dd;
+            }
+            if (r2 < H2) {
+                const double s = sqrt(r2) * invH;
+                const double a = 1.0 - s, b = 0.5 - s;
+                double w = a * a * a;
+                if (b > 0.0) w -= 4.0 * b * b * b;
+                w *= norm;
+                const double mj = m.mass[j];
+                one += w * mj / m.dens[j];
+                sum += mj * w;
+            }
+        }
+        st.advance(g, start, c, xl, hc * hc);
+    }
+    out[i] = sum / one;
+}
+
+// Commit the filtered density (copyFromForce, src/class.h:116-118) and redo
+// what k_reorder derived from it: P/rho^2 and c in the force records (and
+// the FP64 copies and mixed records when present).
+template <int D, class Extra>
+__global__ void k_apply_density(Master m, Packed pk, const double* __restrict__ rho_new, double* __restrict__ pr64,
+                                double* __restrict__ c64, EosConsts e, Extra extra, int64_t n) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double rho = rho_new[p];
+    m.dens[p] = rho;
+    double pr, cs;
+    tait(e, rho, pr, cs);
+    const float4 q = pk.pq[p];
+    pk.pq[p] = make_float4((float)rho, (float)pr, (float)cs, q.w);
+    if (pr64) {
+        pr64[p] = pr;
+        c64[p] = cs;
+    }
+    const float4 x = pk.px[p], v = pk.pv[p];
+    const float xl[3] = {x.x, x.y, x.z}, vf[3] = {v.x, v.y, v.z};
+    extra(p, xl, (int)x.w, vf, rho, pr, cs, m.mass[p]);
 }
 
 // FP64 force pass over neighbour lists (the GPU oracle path): pair
