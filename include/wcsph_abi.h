@@ -37,6 +37,7 @@ extern "C" {
 #define WCSPH_ESTATE 3      /* call out of order (e.g. retrieve without dispatch) */
 #define WCSPH_ENOMEM 4      /* allocation failed or over a stated budget */
 #define WCSPH_EOVERFLOW 5   /* neighbour list or cell grid capacity exceeded */
+#define WCSPH_EIO 6         /* a file could not be opened, read, written or parsed */
 
 /* ---------------- FDPS boundary records (64-bit FDPS build) ----------------
  * 3D sizes: EPI 96 B, EPJ 112 B, RESULT 40 B; 2D: 80 / 96 / 32 B. */
@@ -154,10 +155,13 @@ int wcsph_sim_destroy(wcsph_sim* s);
  * per-particle dt are then produced by wcsph_sim_init. */
 int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const double* dens,
                      const double* mass, const double* h, const int32_t* type);
-/* Pressure, first force pass and gravity (src/main.cpp:69-79, no filter). */
+/* Pressure, first force pass, the Shepard filter when shepard_interval > 0,
+ * and gravity (src/main.cpp:69-79). Resets the clock to time 0, step 0. */
 int wcsph_sim_init(wcsph_sim* s);
-/* nsteps steps of src/main.cpp:89-133 (no Shepard filter), queued on the
- * simulation's stream; returns without waiting. */
+/* nsteps steps of src/main.cpp:89-133, queued on the simulation's stream;
+ * returns without waiting. The Shepard filter runs after the drift of every
+ * step whose index (0 after init) is a multiple of shepard_interval
+ * (src/main.cpp:105, src/param.h:12). */
 int wcsph_sim_step(wcsph_sim* s, int32_t nsteps);
 /* Steps are replayed from a captured two-step hipGraph (one submission per
  * two steps) unless per-step timing is on or graphs are disabled here. */
@@ -233,6 +237,49 @@ typedef struct {
  * per-request timing (latency = end_s - arrival_s) in out[] (same order as
  * reqs). Returns when every request is done. */
 int wcsph_serve(const wcsph_serve_config* cfg, const wcsph_request* reqs, int32_t n, wcsph_request_result* out);
+
+/* ---------------- 4. snapshots and restart ---------------- */
+/* The reference's ASCII snapshot, as FDPS's writeParticleAscii writes it with
+ * FileHeader::writeAscii (src/class.h:45-58) and RealPtcl::writeAscii
+ * (src/class.h:129-135), called from OutputFileWithTimeInterval
+ * (src/io.h:2-37): a line "%e" with the time, a line "%d" with N, then one
+ * line per particle of 12 tab-separated columns, "%ld" for id and tag and
+ * "%.16lf" for mass, x, y, z, vx, vy, vz, dens, snds, pres. In 2D z and vz
+ * are written as 0 and snds is the sound speed; in 3D snds is written as 0.
+ * pos/vel are [dim][n]. NULL id writes 0..n-1; NULL tag, snds or pres write 0. */
+int wcsph_snapshot_write_ascii(const char* path, int32_t dim, int64_t n, double time, const int64_t* id,
+                               const int64_t* tag, const double* mass, const double* pos, const double* vel,
+                               const double* dens, const double* snds, const double* pres);
+/* Reads that format back. With every array NULL it only returns *n and
+ * *time; otherwise each non-NULL array must hold n entries ([dim][n] for
+ * pos/vel). The reference's own reader (RealPtcl::readAscii,
+ * src/class.h:136-142, used by src/io.h:39-46) hands "%.16lf" to fscanf,
+ * which is not a scanf conversion; this reader parses the columns as
+ * written. */
+int wcsph_snapshot_read_ascii(const char* path, int32_t dim, int64_t* n, double* time, int64_t* id, int64_t* tag,
+                              double* mass, double* pos, double* vel, double* dens, double* snds, double* pres);
+/* The simulation's current state in that format, in the original particle
+ * order: id = upload index, tag = type (0 fluid, 1 wall), pressure and sound
+ * speed from the Tait law of the current density (CalcPressure,
+ * src/force.h:3-9), time from the device clock. */
+int wcsph_sim_write_ascii(wcsph_sim* s, const char* path);
+/* Restart file: the full device state after wcsph_sim_init or a step (FP64
+ * SoA in device order with the original indices, acc, div_v, dt_i, the cell
+ * grid, the clock, the Shepard schedule, the next dt, params and config),
+ * native byte order, with a checksum. Replaces the raw RealPtcl dump of
+ * src/io.h:19-30 and InputBinary (src/io.h:60-84), which store EoS
+ * pointers and so cannot be read by another process. */
+int wcsph_sim_save(wcsph_sim* s, const char* path);
+/* Create a simulation on `device` from a restart file. Stepping it continues
+ * bit for bit as the saved simulation would have continued. */
+int wcsph_sim_restore(const char* path, int32_t device, wcsph_sim** out);
+/* Set the clock after wcsph_sim_init (for a state read back from a
+ * snapshot: the reference resumes at the file's time and step + 1,
+ * src/io.h:60-84, and the Shepard schedule follows the step index). */
+int wcsph_sim_set_clock(wcsph_sim* s, double time, int64_t step);
+/* The parameters, configuration and particle count a simulation was created
+ * with (for one made by wcsph_sim_restore); any pointer may be NULL. */
+int wcsph_sim_params(wcsph_sim* s, wcsph_params* p, wcsph_sim_config* cfg, int64_t* n);
 
 #ifdef __cplusplus
 }

@@ -6,9 +6,15 @@ Python mirror of the C-ABI in include/wcsph_abi.h (libwcsph.so):
   multiwalk callback names and argument meaning (src/prototype.h:7-10,
   src/kernel.cu:289,383) and, like them, return an int status;
 * ``Simulation`` drives the device-resident step (cell list, neighbour and
-  force passes, FP64 kick-drift-kick), replacing the loop of
-  src/main.cpp:89-133.
+  force passes, FP64 kick-drift-kick, Shepard filter), replacing the loop of
+  src/main.cpp:89-133;
+* ``write_snapshot_ascii`` / ``read_snapshot_ascii`` and
+  ``Simulation.run_with_output`` write the reference's ASCII snapshots on its
+  schedule (src/io.h:2-37); ``Simulation.save`` / ``Simulation.restore`` are
+  the restart files.
 """
+import os
+
 import ctypes as C
 
 import numpy as np
@@ -18,7 +24,8 @@ from ._abi import (PREC_FP32, PREC_FP64, PREC_MIXED, PRECISIONS, DamBreakDesc, P
 
 __all__ = ["Params", "SimConfig", "WcsphError", "reference_params", "dam_break_params", "init", "finalize",
            "HydrDispatchKernel", "HydrRetrieveKernel", "last_kernel_ms", "epi_dtype", "epj_dtype", "res_dtype",
-           "dam_break_desc", "dam_break_state", "Simulation", "PREC_FP32", "PREC_MIXED", "PREC_FP64"]
+           "dam_break_desc", "dam_break_state", "Simulation", "PREC_FP32", "PREC_MIXED", "PREC_FP64",
+           "write_snapshot_ascii", "read_snapshot_ascii", "snapshot_path", "restart_path"]
 
 
 def reference_params(dim, device=0):
@@ -249,6 +256,118 @@ class Simulation:
         out = np.zeros(3, np.float32)
         check(lib().wcsph_sim_last_force_ms(self._h, ptr(out)))
         return {"binning": float(out[0]), "neighbour": float(out[1]), "force": float(out[2])}
+
+    # -- snapshots and restart ------------------------------------------------
+    def write_ascii(self, path):
+        """The current state as the reference's ASCII snapshot (original
+        order; tag = type; pressure and sound speed from the Tait law)."""
+        check(lib().wcsph_sim_write_ascii(self._h, os.fsencode(path)))
+
+    def save(self, path):
+        """Restart file of the state after init() or step()."""
+        check(lib().wcsph_sim_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def restore(cls, path, device=0):
+        """A simulation resumed from a restart file; stepping it continues bit
+        for bit as the saved one would have."""
+        h = C.c_void_p()
+        check(lib().wcsph_sim_restore(os.fsencode(path), device, C.byref(h)))
+        sim = cls.__new__(cls)
+        sim._h = h
+        params, cfg, n = Params(), SimConfig(), C.c_int64()
+        check(lib().wcsph_sim_params(h, C.byref(params), C.byref(cfg), C.byref(n)))
+        sim.params, sim.cfg, sim.dim, sim.n = params, cfg, params.dim, n.value
+        return sim
+
+    def set_clock(self, time, step):
+        """Clock after init() for a state read back from a snapshot (the
+        reference resumes at the file's time and step + 1, src/io.h:60-84)."""
+        check(lib().wcsph_sim_set_clock(self._h, float(time), int(step)))
+
+    def run_with_output(self, end_time, n_snapshots, outdir, *, restart=True, nproc=1, rank=0):
+        """Step to end_time writing snapshots on the reference's schedule
+        (OutputFileWithTimeInterval, src/io.h:2-37, called at
+        src/main.cpp:79,130): a snapshot after init and after every step that
+        reaches the next output time, which then advances by
+        end_time / n_snapshots. Snapshot k goes to snapshot_path(outdir, k)
+        and, with restart=True, a restart file to restart_path(outdir, k),
+        rotating over 10 like the reference's binary dumps. Steps are queued in
+        chunks of half the estimated steps left to the next output, then one
+        at a time. Returns the number of snapshots written."""
+        os.makedirs(outdir, exist_ok=True)
+        st = self.stats()
+        t_next, k = st["time"], 0
+
+        def output():
+            nonlocal t_next, k
+            self.write_ascii(snapshot_path(outdir, k, nproc, rank))
+            if restart:
+                self.save(restart_path(outdir, k, nproc, rank))
+            t_next += end_time / n_snapshots
+            k += 1
+
+        if st["time"] >= t_next:
+            output()
+        while True:
+            st = self.stats()
+            if st["time"] >= end_time:
+                return k
+            dt = st["dt"] if st["dt"] > 0 else None
+            left = min(t_next, end_time) - st["time"]
+            chunk = int(0.5 * left / dt) if dt else 0
+            self.step(max(chunk, 1))
+            if chunk <= 1:
+                st = self.stats()
+                if st["time"] >= t_next:
+                    output()
+
+
+# ----------------------------------------------------------------------------
+# snapshot files
+# ----------------------------------------------------------------------------
+def snapshot_path(outdir, index, nproc=1, rank=0):
+    """FDPS's per-rank name for snapshot `index` (src/io.h:13-14):
+    outdir/<index>_<nproc>_<rank>.dat, five digits each."""
+    return os.path.join(outdir, f"{index:05d}_{nproc:05d}_{rank:05d}.dat")
+
+
+def restart_path(outdir, index, nproc=1, rank=0):
+    """Name of the restart file written with snapshot `index`, rotating over
+    10 like the reference's binary dumps (src/io.h:19-20)."""
+    return os.path.join(outdir, f"{index % 10:05d}_{nproc:05d}_{rank:05d}.bin")
+
+
+def write_snapshot_ascii(path, time, mass, pos, vel, dens, *, snds=None, pres=None, id=None, tag=None):
+    """Write the reference's ASCII snapshot (FileHeader + RealPtcl::writeAscii,
+    src/class.h:45-58,129-135). pos/vel are [dim][n]."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    dim, n = pos.shape
+    arrs = dict(mass=np.ascontiguousarray(mass, np.float64), pos=pos,
+                vel=np.ascontiguousarray(vel, np.float64).reshape(dim, n),
+                dens=np.ascontiguousarray(dens, np.float64))
+    for k, v, t in (("snds", snds, np.float64), ("pres", pres, np.float64), ("id", id, np.int64),
+                    ("tag", tag, np.int64)):
+        arrs[k] = None if v is None else np.ascontiguousarray(v, t)
+    check(lib().wcsph_snapshot_write_ascii(os.fsencode(path), dim, n, float(time), ptr(arrs["id"]),
+                                           ptr(arrs["tag"]), ptr(arrs["mass"]), ptr(arrs["pos"]), ptr(arrs["vel"]),
+                                           ptr(arrs["dens"]), ptr(arrs["snds"]), ptr(arrs["pres"])))
+
+
+def read_snapshot_ascii(path, dim):
+    """Read a snapshot of that format: dict of time, id, tag, mass, pos
+    [dim][n], vel [dim][n], dens, snds, pres."""
+    n, t = C.c_int64(0), C.c_double(0.0)
+    fn = os.fsencode(path)
+    check(lib().wcsph_snapshot_read_ascii(fn, dim, C.byref(n), C.byref(t), *([None] * 8)))
+    m = n.value
+    out = dict(id=np.zeros(m, np.int64), tag=np.zeros(m, np.int64), mass=np.zeros(m), pos=np.zeros((dim, m)),
+               vel=np.zeros((dim, m)), dens=np.zeros(m), snds=np.zeros(m), pres=np.zeros(m))
+    check(lib().wcsph_snapshot_read_ascii(fn, dim, C.byref(n), C.byref(t),
+                                          *[ptr(out[k]) for k in ("id", "tag", "mass", "pos", "vel", "dens", "snds",
+                                                                  "pres")]))
+    out["time"] = t.value
+    return out
 
 
 # ----------------------------------------------------------------------------

@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libwcsph.so")
 
-WCSPH_OK, WCSPH_EINVAL, WCSPH_EHIP, WCSPH_ESTATE, WCSPH_ENOMEM, WCSPH_EOVERFLOW = 0, 1, 2, 3, 4, 5
+WCSPH_OK, WCSPH_EINVAL, WCSPH_EHIP, WCSPH_ESTATE, WCSPH_ENOMEM, WCSPH_EOVERFLOW, WCSPH_EIO = 0, 1, 2, 3, 4, 5, 6
 PREC_FP32, PREC_MIXED, PREC_FP64 = 0, 1, 2
 PRECISIONS = {"fp32": PREC_FP32, "mixed": PREC_MIXED, "fp64": PREC_FP64}
 
@@ -96,6 +96,8 @@ EXPORTS = [
     "wcsph_sim_force", "wcsph_sim_sync", "wcsph_sim_get_stats", "wcsph_sim_download", "wcsph_sim_neighbours",
     "wcsph_sim_diagnostics", "wcsph_sim_last_force_ms", "wcsph_sim_set_timing", "wcsph_sim_timing",
     "wcsph_sim_grid", "wcsph_sim_set_graphs", "wcsph_sim_query", "wcsph_serve",
+    "wcsph_snapshot_write_ascii", "wcsph_snapshot_read_ascii", "wcsph_sim_write_ascii", "wcsph_sim_save",
+    "wcsph_sim_restore", "wcsph_sim_set_clock", "wcsph_sim_params",
 ]
 
 _lib = None
@@ -145,6 +147,14 @@ def lib():
         "wcsph_sim_query": (C.c_int, [VP, C.POINTER(I32)]),
         "wcsph_serve": (C.c_int, [C.POINTER(ServeConfig), VP, I32, VP]),
         "wcsph_sim_timing": (C.c_int, [VP, VP, C.POINTER(I64)]),
+        "wcsph_snapshot_write_ascii": (C.c_int, [C.c_char_p, I32, I64, F64, VP, VP, VP, VP, VP, VP, VP, VP]),
+        "wcsph_snapshot_read_ascii": (C.c_int, [C.c_char_p, I32, C.POINTER(I64), C.POINTER(F64), VP, VP, VP, VP,
+                                                VP, VP, VP, VP]),
+        "wcsph_sim_write_ascii": (C.c_int, [VP, C.c_char_p]),
+        "wcsph_sim_save": (C.c_int, [VP, C.c_char_p]),
+        "wcsph_sim_restore": (C.c_int, [C.c_char_p, I32, C.POINTER(VP)]),
+        "wcsph_sim_set_clock": (C.c_int, [VP, F64, I64]),
+        "wcsph_sim_params": (C.c_int, [VP, C.POINTER(Params), C.POINTER(SimConfig), C.POINTER(I64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1,12 +1,13 @@
 // Device-resident simulation: host orchestration of the step kernels.
 //
-// One step (src/main.cpp:89-133 without the Shepard filter) is queued as:
+// One step (src/main.cpp:89-133) is queued as:
 //   k_scalars(commit dt)      getTimeStepGlobal result of the previous step
 //   k_kick_drift              initialKick + fullDrift + predict (FP64)
 //   k_bin/k_scan_reduce/k_scan_apply/k_scatter/k_cell_sort   cell list by
 //                             counting sort
 //   k_reorder                 permute the state into cell order + EoS,
 //                             compact FP32 force records (cell-local x)
+//   k_shepard/k_apply_density Shepard filter on steps the schedule names
 //   k_nforce                  fused neighbour search + hydro force (FP32;
 //                             replaces the FDPS tree walk and
 //                             dispatch/retrieve); the FP64 path runs
@@ -56,6 +57,7 @@ struct wcsph_sim {
     double* diag_part = nullptr;
     double* diag_out = nullptr;
     bool uploaded = false, forced = false;
+    bool steppable = false;  // acc, the dt bounds and the clock are those of init or a step
     bool timing = false;
     bool use_graphs = true;
     int64_t hstep = 0;         // steps queued since init (the Shepard filter schedule)
@@ -103,18 +105,46 @@ int64_t sim_bytes(int D, int64_t n, int K, int64_t ncell_guess) {
 
 }  // namespace
 
-static void build_grid(wcsph_sim* s, const double* pos, const double* h) {
-    const int D = s->D;
+// Largest smoothing length (as FP32, the precision of the predicate) and
+// whether all are equal.
+static void scan_h(wcsph_sim* s, const double* h) {
     const int64_t n = s->n;
     double hmax = 0.0;
     for (int64_t i = 0; i < n; ++i) hmax = std::max(hmax, (double)(float)h[i]);
     s->uniform_h = true;
     for (int64_t i = 0; i < n; ++i) s->uniform_h = s->uniform_h && (float)h[i] == (float)h[0];
     WCSPH_REQUIRE(hmax > 0.0, WCSPH_EINVAL, "smoothing lengths must be positive");
+    s->hmax = (float)hmax;
+}
+
+static void alloc_grid(wcsph_sim* s, const double* lo, double L, const int32_t* nc) {
+    s->grid.invL = 1.0 / L;
+    s->grid.L = L;
+    s->grid.Lf = (float)L;
+    int64_t ncell = 1;
+    for (int d = 0; d < 3; ++d) {
+        s->grid.lo[d] = d < s->D ? lo[d] : 0.0;
+        s->grid.nc[d] = d < s->D ? nc[d] : 1;
+        WCSPH_REQUIRE(s->grid.nc[d] >= 1, WCSPH_EINVAL, "cell grid needs at least one cell per axis");
+        ncell *= s->grid.nc[d];
+    }
+    WCSPH_REQUIRE(ncell < (int64_t)1 << 31, WCSPH_EOVERFLOW, "cell grid exceeds 2^31 cells");
+    s->ncell = ncell;
+    s->npart = ceil_div(ncell, kScanTile);
+    s->count = s->alloc<uint32_t>(ncell);
+    s->start = s->alloc<uint32_t>(ncell + 1);
+    s->part = s->alloc<uint32_t>(s->npart);
+    // the scan clears count after reading it, ready for the next pass
+    WCSPH_HIP(hipMemsetAsync(s->count, 0, sizeof(uint32_t) * ncell, s->st));
+}
+
+static void build_grid(wcsph_sim* s, const double* pos, const double* h) {
+    const int D = s->D;
+    const int64_t n = s->n;
+    scan_h(s, h);
     // cell edge slightly above the largest support 2 h_max, so any pair that
     // passes the FP32 predicate lies in adjacent cells
-    const double L = 2.0 * hmax * (1.0 + 1.0e-5) / s->R;
-    s->hmax = (float)hmax;
+    const double L = 2.0 * (double)s->hmax * (1.0 + 1.0e-5) / s->R;
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     bool have_box = true;
     for (int d = 0; d < D; ++d) have_box = have_box && s->cfg.box_hi[d] > s->cfg.box_lo[d];
@@ -133,23 +163,28 @@ static void build_grid(wcsph_sim* s, const double* pos, const double* h) {
             hi[d] += 2.0 * L + (d == D - 1 ? 0.5 * ext : 0.0);  // room above for splashes
         }
     }
-    s->grid.invL = 1.0 / L;
-    s->grid.L = L;
-    s->grid.Lf = (float)L;
-    int64_t ncell = 1;
-    for (int d = 0; d < 3; ++d) {
-        s->grid.lo[d] = d < D ? lo[d] : 0.0;
-        s->grid.nc[d] = d < D ? (int)std::max<int64_t>(1, (int64_t)std::ceil((hi[d] - lo[d]) / L)) : 1;
-        ncell *= s->grid.nc[d];
-    }
-    WCSPH_REQUIRE(ncell < (int64_t)1 << 31, WCSPH_EOVERFLOW, "cell grid exceeds 2^31 cells");
-    s->ncell = ncell;
-    s->npart = ceil_div(ncell, kScanTile);
-    s->count = s->alloc<uint32_t>(ncell);
-    s->start = s->alloc<uint32_t>(ncell + 1);
-    s->part = s->alloc<uint32_t>(s->npart);
-    // the scan clears count after reading it, ready for the next pass
-    WCSPH_HIP(hipMemsetAsync(s->count, 0, sizeof(uint32_t) * ncell, s->st));
+    int32_t nc[3] = {1, 1, 1};
+    for (int d = 0; d < D; ++d) nc[d] = (int32_t)std::max<int64_t>(1, (int64_t)std::ceil((hi[d] - lo[d]) / L));
+    alloc_grid(s, lo, L, nc);
+}
+
+// Checks and constants of the precision path that depend on the grid and h.
+static void setup_precision(wcsph_sim* s) {
+    if (s->cfg.precision != WCSPH_PREC_MIXED) return;
+    const int D = s->D;
+    WCSPH_REQUIRE(s->uniform_h, WCSPH_EINVAL, "the mixed-precision path needs one smoothing length for all particles");
+    WCSPH_REQUIRE(s->grid.nc[0] <= 2048, WCSPH_EINVAL,
+                  "the mixed-precision path needs at most 2048 cells along x (FP16 cell index)");
+    WCSPH_REQUIRE(mixed_records(s->n) <= ((int64_t)1 << kMixRecBits), WCSPH_EINVAL,
+                  "the mixed-precision path holds at most 2^26 particles");
+    const double hh = (double)s->hmax;
+    s->mc.inv_h = (float)(1.0 / hh);
+    s->mc.inv_c0 = (float)(1.0 / s->p.c0);
+    s->mc.inv_rho0 = (float)(1.0 / s->p.rho0);
+    s->mc.pr_scale = (float)(s->p.rho0 / (s->p.c0 * s->p.c0));
+    const double cnorm = (D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI) / (D == 2 ? 8.0 : 16.0);
+    s->mc.m_scale = (float)(cnorm / (s->p.rho0 * std::pow(hh, D)));
+    s->mc.half_Ls = (float)(0.5 * s->grid.L / hh);
 }
 
 static void bin_and_reorder(wcsph_sim* s) {
@@ -589,21 +624,7 @@ int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const d
             WCSPH_REQUIRE(type[i] == 0 || type[i] == 1, WCSPH_EINVAL, "type must be 0 (fluid) or 1 (wall)");
         }
         build_grid(s, pos, h);
-        if (s->cfg.precision == WCSPH_PREC_MIXED) {
-            WCSPH_REQUIRE(s->uniform_h, WCSPH_EINVAL, "the mixed-precision path needs one smoothing length for all particles");
-            WCSPH_REQUIRE(s->grid.nc[0] <= 2048, WCSPH_EINVAL,
-                          "the mixed-precision path needs at most 2048 cells along x (FP16 cell index)");
-            WCSPH_REQUIRE(mixed_records(n) <= ((int64_t)1 << kMixRecBits), WCSPH_EINVAL,
-                          "the mixed-precision path holds at most 2^26 particles");
-            const double hh = (double)s->hmax;
-            s->mc.inv_h = (float)(1.0 / hh);
-            s->mc.inv_c0 = (float)(1.0 / s->p.c0);
-            s->mc.inv_rho0 = (float)(1.0 / s->p.rho0);
-            s->mc.pr_scale = (float)(s->p.rho0 / (s->p.c0 * s->p.c0));
-            const double cnorm = (D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI) / (D == 2 ? 8.0 : 16.0);
-            s->mc.m_scale = (float)(cnorm / (s->p.rho0 * std::pow(hh, D)));
-            s->mc.half_Ls = (float)(0.5 * s->grid.L / hh);
-        }
+        setup_precision(s);
         std::vector<uint8_t> t8(n);
         std::vector<uint32_t> id(n);
         for (int64_t i = 0; i < n; ++i) {
@@ -633,6 +654,7 @@ int wcsph_sim_init(wcsph_sim* s) {
         if (s->p.shepard_interval > 0) shepard_pass(s);  // after the first force (src/main.cpp:75)
         final_stage(s, 0);
         s->hstep = 0;
+        s->steppable = true;
         WCSPH_HIP(hipGetLastError());
     });
 }
@@ -644,6 +666,9 @@ static hipGraphExec_t two_step_graph(wcsph_sim* s) {
     hipGraphExec_t& ge = s->graph2[s->cur];
     if (ge) return ge;
     const int cur0 = s->cur;
+    // buffers allocated on first use must exist before capture (a restored
+    // FP64 simulation has not yet run the pass that allocates its lists)
+    if (s->cfg.precision == WCSPH_PREC_FP64) ensure_lists(s);
     hipGraph_t g = nullptr;
     WCSPH_HIP(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
     one_step(s, false);
@@ -658,7 +683,8 @@ static hipGraphExec_t two_step_graph(wcsph_sim* s) {
 
 int wcsph_sim_step(wcsph_sim* s, int32_t nsteps) {
     return guarded([&] {
-        WCSPH_REQUIRE(s && s->uploaded && s->forced, WCSPH_ESTATE, "call wcsph_sim_init first");
+        WCSPH_REQUIRE(s && s->uploaded && s->steppable, WCSPH_ESTATE,
+                      "call wcsph_sim_init first (and again after wcsph_sim_force)");
         WCSPH_REQUIRE(nsteps >= 0, WCSPH_EINVAL, "nsteps < 0");
         WCSPH_HIP(hipSetDevice(s->device));
         // pairs of steps without a filter replay the captured graph
@@ -703,6 +729,7 @@ int wcsph_sim_force(wcsph_sim* s) {
         WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "upload a state first");
         WCSPH_HIP(hipSetDevice(s->device));
         hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
+        s->steppable = false;  // acc is hydro only and the dt bounds are stale
         WCSPH_HIP(hipEventRecord(s->ev[0], s->st));
         bin_and_reorder(s);
         neigh_force(s, s->ev[1], s->ev[2], s->ev[3]);
@@ -860,6 +887,310 @@ int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double xwall
         WCSPH_HIP(hipGetLastError());
         WCSPH_HIP(hipMemcpyAsync(out, s->diag_out, sizeof(double) * kDiag, hipMemcpyDeviceToHost, s->st));
         WCSPH_HIP(hipStreamSynchronize(s->st));
+    });
+}
+
+}  // extern "C"
+
+// ---------------- snapshots and restart ----------------
+namespace {
+
+constexpr char kRestartMagic[8] = {'W', 'C', 'S', 'P', 'H', 'R', 'S', 'T'};
+constexpr uint32_t kRestartVersion = 1;
+
+// Restart file header; the payload follows it (see restart_arrays).
+struct RestartHeader {
+    char magic[8];
+    uint32_t version, header_bytes;
+    int64_t n;
+    wcsph_params params;
+    wcsph_sim_config cfg;
+    double grid_lo[3], grid_L;
+    int32_t grid_nc[3], dim;
+    double time, dt, dt_next;
+    int64_t step, hstep;
+    uint64_t pairs_last, pairs_total;
+    uint32_t escaped, maxcnt, overflow, pad;
+    uint64_t payload_bytes, checksum;
+};
+
+// The host-side copy of a simulation's state, in device (cell) order.
+struct HostState {
+    std::vector<double> pos, vel, dens, mass, h, acc, divv, dti;
+    std::vector<uint32_t> id;
+    std::vector<uint8_t> type;
+    void resize(int D, int64_t n) {
+        pos.resize((size_t)D * n);
+        vel.resize((size_t)D * n);
+        acc.resize((size_t)D * n);
+        for (auto* v : {&dens, &mass, &h, &divv, &dti}) v->resize((size_t)n);
+        id.resize((size_t)n);
+        type.resize((size_t)n);
+    }
+};
+
+// Payload order: pos, vel, dens, mass, h, acc, div_v, dt_i (FP64), id (u32), type (u8).
+template <class F>
+void restart_arrays(HostState& h, F&& f) {
+    for (auto* v : {&h.pos, &h.vel, &h.dens, &h.mass, &h.h, &h.acc, &h.divv, &h.dti})
+        f((void*)v->data(), v->size() * sizeof(double));
+    f((void*)h.id.data(), h.id.size() * sizeof(uint32_t));
+    f((void*)h.type.data(), h.type.size());
+}
+
+int64_t restart_payload_bytes(int D, int64_t n) { return (int64_t)(3 * D + 5) * 8 * n + 5 * n; }
+
+// FNV-1a over 64-bit little-endian words (tail bytes zero-padded).
+uint64_t fnv1a_words(uint64_t hs, const void* data, size_t bytes) {
+    const uint8_t* p = (const uint8_t*)data;
+    size_t i = 0;
+    for (; i + 8 <= bytes; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + i, 8);
+        hs = (hs ^ w) * 0x100000001b3ull;
+    }
+    if (i < bytes) {
+        uint64_t w = 0;
+        std::memcpy(&w, p + i, bytes - i);
+        hs = (hs ^ w) * 0x100000001b3ull;
+    }
+    return hs;
+}
+
+struct FileCloser {
+    void operator()(FILE* f) const {
+        if (f) std::fclose(f);
+    }
+};
+
+void fetch_state(wcsph_sim* s, HostState& hs) {
+    const int D = s->D;
+    const int64_t n = s->n;
+    const Master& m = s->m[s->cur];
+    hs.resize(D, n);
+    auto get = [&](void* dst, const void* src, size_t bytes) {
+        WCSPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    };
+    get(hs.pos.data(), m.pos, sizeof(double) * D * n);
+    get(hs.vel.data(), m.vel, sizeof(double) * D * n);
+    get(hs.dens.data(), m.dens, sizeof(double) * n);
+    get(hs.mass.data(), m.mass, sizeof(double) * n);
+    get(hs.h.data(), m.h, sizeof(double) * n);
+    get(hs.acc.data(), s->acc, sizeof(double) * D * n);
+    get(hs.divv.data(), s->divv, sizeof(double) * n);
+    get(hs.dti.data(), s->dti, sizeof(double) * n);
+    get(hs.id.data(), m.id, sizeof(uint32_t) * n);
+    get(hs.type.data(), m.type, n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int wcsph_sim_set_clock(wcsph_sim* s, double time, int64_t step) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && s->uploaded && s->steppable, WCSPH_ESTATE, "call wcsph_sim_init first");
+        WCSPH_REQUIRE(step >= 0 && std::isfinite(time), WCSPH_EINVAL, "step must be >= 0 and time finite");
+        WCSPH_HIP(hipSetDevice(s->device));
+        Scalars h{};
+        WCSPH_HIP(hipMemcpyAsync(&h, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, s->st));
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+        h.time = time;
+        h.step = step;
+        WCSPH_HIP(hipMemcpyAsync(s->sc, &h, sizeof(Scalars), hipMemcpyHostToDevice, s->st));
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+        s->hstep = step;
+    });
+}
+
+int wcsph_sim_params(wcsph_sim* s, wcsph_params* p, wcsph_sim_config* cfg, int64_t* n) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s, WCSPH_EINVAL, "null simulation");
+        if (p) *p = s->p;
+        if (cfg) *cfg = s->cfg;
+        if (n) *n = s->n;
+    });
+}
+
+int wcsph_sim_write_ascii(wcsph_sim* s, const char* path) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && s->uploaded && path, WCSPH_EINVAL, "null argument or nothing uploaded");
+        WCSPH_HIP(hipSetDevice(s->device));
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+        HostState hs;
+        fetch_state(s, hs);
+        Scalars sc{};
+        WCSPH_HIP(hipMemcpy(&sc, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+        const int D = s->D;
+        const int64_t n = s->n;
+        std::vector<double> pos((size_t)D * n), vel((size_t)D * n), dens(n), mass(n), snds(n), pres(n);
+        std::vector<int64_t> id(n), tag(n);
+        const double rho0 = s->p.rho0, gamma = s->p.gamma;
+        const double B = s->p.c0 * s->p.c0 * rho0 / gamma;  // src/EoS.h:78
+        parallel_for(n, [&](int64_t q) {
+            const int64_t i = hs.id[q];
+            for (int d = 0; d < D; ++d) {
+                pos[d * n + i] = hs.pos[d * n + q];
+                vel[d * n + i] = hs.vel[d * n + q];
+            }
+            const double rho = hs.dens[q];
+            dens[i] = rho;
+            mass[i] = hs.mass[q];
+            // src/EoS.h:80-86
+            pres[i] = std::max(B * (std::pow(rho / rho0, gamma) - 1.0), 0.0);
+            snds[i] = std::sqrt(std::max(B * gamma / rho0 * std::pow(rho / rho0, gamma - 1.0), 1.0e-16));
+            id[i] = i;
+            tag[i] = hs.type[q];
+        });
+        const int rc = wcsph_snapshot_write_ascii(path, D, n, sc.time, id.data(), tag.data(), mass.data(),
+                                                  pos.data(), vel.data(), dens.data(), snds.data(), pres.data());
+        if (rc != WCSPH_OK) throw Error{rc, wcsph_last_error()};
+    });
+}
+
+int wcsph_sim_save(wcsph_sim* s, const char* path) {
+    return guarded([&] {
+        WCSPH_REQUIRE(s && path, WCSPH_EINVAL, "null argument");
+        WCSPH_REQUIRE(s->uploaded && s->steppable, WCSPH_ESTATE,
+                      "a restart file holds the state after wcsph_sim_init or a step");
+        WCSPH_HIP(hipSetDevice(s->device));
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+        HostState hs;
+        fetch_state(s, hs);
+        Scalars sc{};
+        WCSPH_HIP(hipMemcpy(&sc, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+        std::vector<double> dtm((size_t)s->P.nblk);
+        WCSPH_HIP(hipMemcpy(dtm.data(), s->P.dtmin, sizeof(double) * s->P.nblk, hipMemcpyDeviceToHost));
+        RestartHeader hd;
+        std::memset(&hd, 0, sizeof hd);
+        std::memcpy(hd.magic, kRestartMagic, 8);
+        hd.version = kRestartVersion;
+        hd.header_bytes = sizeof(RestartHeader);
+        hd.n = s->n;
+        hd.params = s->p;
+        hd.cfg = s->cfg;
+        for (int d = 0; d < 3; ++d) {
+            hd.grid_lo[d] = s->grid.lo[d];
+            hd.grid_nc[d] = s->grid.nc[d];
+        }
+        hd.grid_L = s->grid.L;
+        hd.dim = s->D;
+        hd.time = sc.time;
+        hd.dt = sc.dt;
+        hd.dt_next = *std::min_element(dtm.begin(), dtm.end());
+        hd.step = sc.step;
+        hd.hstep = s->hstep;
+        hd.pairs_last = sc.pairs_last;
+        hd.pairs_total = sc.pairs_total;
+        hd.escaped = sc.escaped;
+        hd.maxcnt = sc.maxcnt;
+        hd.overflow = sc.overflow;
+        hd.payload_bytes = (uint64_t)restart_payload_bytes(s->D, s->n);
+        uint64_t ck = 0xcbf29ce484222325ull;
+        restart_arrays(hs, [&](void* p, size_t b) { ck = fnv1a_words(ck, p, b); });
+        hd.checksum = ck;
+        // written beside the target and renamed, so a failed save leaves any
+        // earlier file at path intact
+        const std::string tmp = std::string(path) + ".part";
+        {
+            std::unique_ptr<FILE, FileCloser> f(std::fopen(tmp.c_str(), "wb"));
+            WCSPH_REQUIRE(f != nullptr, WCSPH_EIO, "cannot open " + tmp + ": " + std::strerror(errno));
+            bool ok = std::fwrite(&hd, sizeof hd, 1, f.get()) == 1;
+            restart_arrays(hs, [&](void* p, size_t b) { ok = ok && std::fwrite(p, 1, b, f.get()) == b; });
+            ok = ok && std::fflush(f.get()) == 0;
+            if (!ok) {
+                f.reset();
+                std::remove(tmp.c_str());
+                throw Error{WCSPH_EIO, "write failed: " + tmp};
+            }
+        }
+        WCSPH_REQUIRE(std::rename(tmp.c_str(), path) == 0, WCSPH_EIO,
+                      std::string("cannot rename ") + tmp + " to " + path + ": " + std::strerror(errno));
+    });
+}
+
+int wcsph_sim_restore(const char* path, int32_t device, wcsph_sim** out) {
+    return guarded([&] {
+        WCSPH_REQUIRE(path && out, WCSPH_EINVAL, "null argument");
+        *out = nullptr;
+        std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "rb"));
+        WCSPH_REQUIRE(f != nullptr, WCSPH_EIO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+        RestartHeader hd;
+        WCSPH_REQUIRE(std::fread(&hd, sizeof hd, 1, f.get()) == 1, WCSPH_EIO, "restart file shorter than its header");
+        WCSPH_REQUIRE(std::memcmp(hd.magic, kRestartMagic, 8) == 0, WCSPH_EIO, "not a restart file (bad magic)");
+        WCSPH_REQUIRE(hd.version == kRestartVersion && hd.header_bytes == sizeof(RestartHeader), WCSPH_EIO,
+                      "restart file of another version or layout");
+        WCSPH_REQUIRE((hd.dim == 2 || hd.dim == 3) && hd.params.dim == hd.dim && hd.n > 0 &&
+                          hd.n < ((int64_t)1 << 31),
+                      WCSPH_EIO, "restart header: bad dim or particle count");
+        WCSPH_REQUIRE(hd.payload_bytes == (uint64_t)restart_payload_bytes(hd.dim, hd.n), WCSPH_EIO,
+                      "restart header: payload size does not match the particle count");
+        WCSPH_REQUIRE(std::fseek(f.get(), 0, SEEK_END) == 0, WCSPH_EIO, "cannot seek");
+        const long size = std::ftell(f.get());
+        WCSPH_REQUIRE(size == (long)(sizeof hd + hd.payload_bytes), WCSPH_EIO,
+                      "restart file truncated or padded: " + std::to_string(size) + " bytes, expected " +
+                          std::to_string(sizeof hd + hd.payload_bytes));
+        WCSPH_REQUIRE(std::fseek(f.get(), (long)sizeof hd, SEEK_SET) == 0, WCSPH_EIO, "cannot seek");
+        const int D = hd.dim;
+        const int64_t n = hd.n;
+        HostState hs;
+        hs.resize(D, n);
+        bool ok = true;
+        restart_arrays(hs, [&](void* p, size_t b) { ok = ok && std::fread(p, 1, b, f.get()) == b; });
+        WCSPH_REQUIRE(ok, WCSPH_EIO, "restart payload read failed");
+        uint64_t ck = 0xcbf29ce484222325ull;
+        restart_arrays(hs, [&](void* p, size_t b) { ck = fnv1a_words(ck, p, b); });
+        WCSPH_REQUIRE(ck == hd.checksum, WCSPH_EIO, "restart file checksum mismatch");
+        std::vector<uint8_t> seen((size_t)n, 0);
+        for (int64_t q = 0; q < n; ++q) {
+            WCSPH_REQUIRE(hs.id[q] < (uint64_t)n && !seen[hs.id[q]], WCSPH_EIO, "restart file: ids are not a permutation");
+            seen[hs.id[q]] = 1;
+            WCSPH_REQUIRE(hs.type[q] <= 1, WCSPH_EIO, "restart file: bad particle type");
+        }
+        wcsph_params p = hd.params;
+        p.device = device;
+        wcsph_sim* raw = nullptr;
+        const int rc = wcsph_sim_create(&p, &hd.cfg, n, &raw);
+        if (rc != WCSPH_OK) throw Error{rc, wcsph_last_error()};
+        std::unique_ptr<wcsph_sim> s(raw);
+        WCSPH_HIP(hipSetDevice(s->device));
+        scan_h(s.get(), hs.h.data());
+        alloc_grid(s.get(), hd.grid_lo, hd.grid_L, hd.grid_nc);
+        setup_precision(s.get());
+        const Master& m = s->m[s->cur];
+        auto put = [&](void* dst, const void* src, size_t bytes) {
+            WCSPH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        };
+        put(m.pos, hs.pos.data(), sizeof(double) * D * n);
+        put(m.vel, hs.vel.data(), sizeof(double) * D * n);
+        put(m.vh, hs.vel.data(), sizeof(double) * D * n);
+        put(m.dens, hs.dens.data(), sizeof(double) * n);
+        put(m.mass, hs.mass.data(), sizeof(double) * n);
+        put(m.h, hs.h.data(), sizeof(double) * n);
+        put(s->acc, hs.acc.data(), sizeof(double) * D * n);
+        put(s->divv, hs.divv.data(), sizeof(double) * n);
+        put(s->dti, hs.dti.data(), sizeof(double) * n);
+        put(m.id, hs.id.data(), sizeof(uint32_t) * n);
+        put(m.type, hs.type.data(), n);
+        Scalars sc{};
+        sc.dt = hd.dt;
+        sc.time = hd.time;
+        std::memcpy(&sc.dtmin_bits, &hd.dt, 8);
+        sc.step = hd.step;
+        sc.pairs_last = hd.pairs_last;
+        sc.pairs_total = hd.pairs_total;
+        sc.escaped = hd.escaped;
+        sc.maxcnt = hd.maxcnt;
+        sc.overflow = hd.overflow;
+        put(s->sc, &sc, sizeof sc);
+        // the next step's k_scalars takes the minimum of the per-block bounds
+        std::vector<double> dtm((size_t)s->P.nblk, 1.0e30);
+        dtm[0] = hd.dt_next;
+        put(s->P.dtmin, dtm.data(), sizeof(double) * s->P.nblk);
+        WCSPH_HIP(hipStreamSynchronize(s->st));
+        s->hstep = hd.hstep;
+        s->uploaded = s->steppable = true;  // neighbour lists need a force pass first
+        *out = s.release();
     });
 }
 
