@@ -316,7 +316,7 @@ def main():
     K = args.steps
     dtype = {"fp32": "fp32", "mixed": "fp16x2/fp32acc", "fp64": "fp64"}[args.precision]
     force_ms = head["force_ms_per_launch"]
-    achieved = FLOP_PER_PAIR[dim] * head["pairs_per_step"] / (force_ms * 1e-3) / 1e12
+    achieved = FLOP_PER_PAIR[dim] * head["force_pairs_per_launch"] / (force_ms * 1e-3) / 1e12
     kname = {"fp32": "k_nforce", "mixed": "k_nforce_mixed", "fp64": "k_force64"}[args.precision]
     line = {
         "metric": "pair-interactions/s",
@@ -351,7 +351,8 @@ def main():
     if dim == 3 and abs(dx - 0.0126) < 1e-12:
         prof = profiled_traffic(kname)
         if prof:
-            line["roofline"]["traffic_profiled"] = prof
+            line["roofline"]["traffic"] = prof["bytes_per_launch"]
+            line["roofline"]["traffic_source"] = prof["source"]
     line["hbm_copy_GBps"] = W.copy_probe_gbps(1 << 30, 10)
     if rank == 0 and args.cpu_seconds > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -369,8 +370,12 @@ def main():
 
 
 def run_path(W, params, st, prec, args):
-    """Warm a simulation up, then time args.steps full steps (HIP events on
-    the simulation stream per stage). Returns the simulation and its figures."""
+    """Warm a simulation up, then time args.steps full steps as a user runs
+    them (two-step hipGraph replays, wall clock between synchronisations).
+    A second window of up to 100 steps, launched one by one with HIP events
+    around each stage on the simulation stream, gives the per-stage device
+    times and the force kernel's launch duration. Returns the simulation and
+    its figures."""
     sim = W.Simulation.from_state(params, st, precision=prec, cells_per_support=args.cells_per_support,
                                   queue_depth=args.queue_depth)
     sim.initial_state = st
@@ -380,27 +385,33 @@ def run_path(W, params, st, prec, args):
     sim.sync()
     print(f"# {prec}: warmup {args.warmup} steps {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
     s0 = sim.stats()
-    sim.set_timing(True)
-    sim.sync()
     t0 = time.perf_counter()
     sim.step(args.steps)
     sim.sync()
     t1 = time.perf_counter()
     s1 = sim.stats()
-    tm = sim.timing()
-    sim.set_timing(False)
     K = args.steps
     wall = t1 - t0
     pairs = s1["pairs_total"] - s0["pairs_total"]
-    force_ms = tm["force"] / K
+    # stage window (events disable the graph replays)
+    Kt = max(2, min(K, 100))
+    sim.set_timing(True)
+    sim.step(Kt)
+    tm = sim.timing()
+    s2 = sim.stats()
+    sim.set_timing(False)
+    pairs_t = s2["pairs_total"] - s1["pairs_total"]
+    force_ms = tm["force"] / Kt
     return sim, {
         "value": pairs / wall,
         "ms_per_step": 1e3 * wall / K,
         "steps_per_s": K / wall,
         "pairs_per_step": pairs / K,
         "force_ms_per_launch": force_ms,
-        "force_kernel_pairs_per_s": pairs / K / (force_ms * 1e-3),
-        "stage_ms_per_step": {k: tm[k] / K for k in ("kick_bin_reorder", "neighbour", "force", "final", "step")},
+        "force_pairs_per_launch": pairs_t / Kt,
+        "force_kernel_pairs_per_s": pairs_t / Kt / (force_ms * 1e-3),
+        "stage_window_steps": Kt,
+        "stage_ms_per_step": {k: tm[k] / Kt for k in ("kick_bin_reorder", "neighbour", "force", "final", "step")},
         "sim_time_s": s1["time"],
         "escaped": s1["escaped"],
         "overflow": s1["overflow"],

@@ -53,6 +53,9 @@ struct Mixed {
 
 __host__ __device__ inline int64_t mixed_records(int64_t n) { return (n + 1) / 2; }
 
+__device__ __forceinline__ h2 as_h2(uint32_t u) { return __builtin_bit_cast(h2, u); }
+__device__ __forceinline__ uint32_t as_u32(h2 v) { return __builtin_bit_cast(uint32_t, v); }
+
 struct MixedConsts {
     float inv_h, inv_c0, inv_rho0, pr_scale, m_scale;  // SI -> scaled units; m_scale includes C_D / 2^(D+1)
     float half_Ls;  // half the cell edge in h: positions are stored relative to the cell centre
@@ -81,13 +84,52 @@ __device__ __forceinline__ void write_mixed(const Mixed& mx, const MixedConsts& 
     S[6] = (_Float16)0.f;
 }
 
+// The same records written whole: lanes p and p ^ 1 (one wave) swap their
+// halves and the even lane stores the three 16-byte records. A lane past
+// the end supplies zeros. Every lane of the wave must call it.
 template <int D>
-struct MixedExtra {  // k_reorder hook writing the pair records
+__device__ __forceinline__ void write_mixed_pair(const Mixed& mx, const MixedConsts& mc, int64_t p, bool live,
+                                                 const float* xl, int cx, const float* vf, double rho, double pr,
+                                                 double c, double m) {
+    uint32_t mine[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    if (live) {
+        const h2 v[6] = {
+            h2{(_Float16)(xl[0] * mc.inv_h - mc.half_Ls), (_Float16)(xl[1] * mc.inv_h - mc.half_Ls)},
+            h2{(_Float16)(D == 3 ? xl[2] * mc.inv_h - mc.half_Ls : 0.f), (_Float16)(float)cx},
+            h2{(_Float16)vf[0], (_Float16)vf[1]},
+            h2{(_Float16)(D == 3 ? vf[2] : 0.f), (_Float16)(float)(m * mc.m_scale)},
+            h2{(_Float16)(float)(rho * mc.inv_rho0), (_Float16)(float)(pr * mc.pr_scale)},
+            h2{(_Float16)(float)(0.5 * c * mc.inv_c0), (_Float16)0.f}};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) mine[k] = as_u32(v[k]);
+    }
+    uint32_t other[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) other[k] = (uint32_t)__shfl_xor((int)mine[k], 1, 64);
+    if ((p & 1) == 0 && live) {
+        // record half 0 = this lane (even p), half 1 = the partner
+        auto lo = [](uint32_t a, uint32_t b) { return h2{as_h2(a).x, as_h2(b).x}; };
+        auto hi = [](uint32_t a, uint32_t b) { return h2{as_h2(a).y, as_h2(b).y}; };
+        mx.pos[p >> 1] = PosRec16{lo(mine[0], other[0]), hi(mine[0], other[0]), lo(mine[1], other[1]),
+                                  hi(mine[1], other[1])};
+        mx.vel[p >> 1] = VelRec16{lo(mine[2], other[2]), hi(mine[2], other[2]), lo(mine[3], other[3]),
+                                  hi(mine[3], other[3])};
+        mx.st[p >> 1] = StateRec16{lo(mine[4], other[4]), hi(mine[4], other[4]), lo(mine[5], other[5]),
+                                   hi(mine[5], other[5])};
+    }
+}
+
+template <int D>
+struct MixedExtra {  // k_reorder / k_apply_density hook writing the pair records
     Mixed mx;
     MixedConsts mc;
     __device__ void operator()(int64_t p, const float* xl, int cx, const float* vf, double rho, double pr, double c,
                                double m) const {
         write_mixed<D>(mx, mc, p, xl, cx, vf, rho, pr, c, m);
+    }
+    __device__ void store_pair(int64_t p, bool live, const float* xl, int cx, const float* vf, double rho,
+                               double pr, double c, double m) const {
+        write_mixed_pair<D>(mx, mc, p, live, xl, cx, vf, rho, pr, c, m);
     }
 };
 
@@ -104,8 +146,6 @@ __device__ __forceinline__ h2 hrsq(h2 v) {
 __device__ __forceinline__ h2 hmax(h2 a, h2 b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ h2 hmin(h2 a, h2 b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ h2 hfma(h2 a, h2 b, h2 c) { return __builtin_elementwise_fma(a, b, c); }
-__device__ __forceinline__ h2 as_h2(uint32_t u) { return __builtin_bit_cast(h2, u); }
-__device__ __forceinline__ uint32_t as_u32(h2 v) { return __builtin_bit_cast(uint32_t, v); }
 
 struct NFMixedArgs {
     Mixed mx;
@@ -125,6 +165,8 @@ struct NFMixedArgs {
     double* dti;
     Partials P;
     Scalars* s;
+    int fused_final;  // run the final stage (FinalArgs) in the epilogue
+    FinalArgs fin;
 };
 
 // Tuning constants (build-time overridable): ring queue entries per lane
@@ -347,19 +389,25 @@ void k_nforce_mixed(NFMixedArgs a) {
         }
     }
     while (__any(qn > 0u)) drain();
+    double cand = 1.0e30;
     if (live) {
         const int64_t n = a.n;
         const float4 qi4 = a.pq[i];  // FP32 rho, P/rho^2, c, h
         const double c0 = a.c0, h = a.h;
         const double ks = c0 * c0 / h, kv = 4.0 * a.nu * c0 / (h * h);
+        double ad[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) a.acc[d * n + i] = ks * accp[d] + kv * accv[d];
+        for (int d = 0; d < D; ++d) ad[d] = ks * accp[d] + kv * accv[d];
         a.divv[i] = -(c0 / h) * divv * (1.0 / ((double)qi4.x * (double)a.inv_rho0));
         const float vs = fmaxf((float)vsig2.x, (float)vsig2.y);
-        a.dti[i] = a.c_cfl * h / ((double)qi4.z + (double)vs * c0);
+        const double dti = a.c_cfl * h / ((double)qi4.z + (double)vs * c0);
+        a.dti[i] = dti;
+        if (a.fused_final) cand = final_particle<D>(a.fin, a.s, i, n, ad, dti);
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.acc[d * n + i] = ad[d];
         --total;  // the particle itself passed the predicate (r = 0) and added zero
     }
-    block_count_atomic(total, a.s, true);
+    block_count_atomic(total, a.s, true, a.fused_final ? &a.P.dtmin[blockIdx.x] : nullptr, cand);
 }
 
 }  // namespace wcsph

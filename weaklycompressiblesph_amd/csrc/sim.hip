@@ -2,19 +2,20 @@
 //
 // One step (src/main.cpp:89-133) is queued as:
 //   k_scalars(commit dt)      getTimeStepGlobal result of the previous step
-//   k_kick_drift              initialKick + fullDrift + predict (FP64)
-//   k_bin/k_scan_reduce/k_scan_apply/k_scatter/k_cell_sort   cell list by
-//                             counting sort
-//   k_reorder                 permute the state into cell order + EoS,
-//                             compact FP32 force records (cell-local x)
+//   k_bin                     initialKick + fullDrift (FP64), then the cell
+//                             key and slot of each particle (counting sort)
+//   k_scan_reduce/k_scan_apply/k_scatter   cell starts and cell membership
+//   k_reorder                 permute the state into cell order (ranked by
+//                             previous index within each cell), predict,
+//                             EoS, compact FP32 / FP16 force records
 //   k_shepard/k_apply_density Shepard filter on steps the schedule names
-//   k_nforce                  fused neighbour search + hydro force (FP32;
-//                             replaces the FDPS tree walk and
-//                             dispatch/retrieve); the FP64 path runs
-//                             k_neigh (lists) + k_force64 instead
-//   k_final                   gravity + finalKick + density floor + dt bound
+//   k_nforce / k_nforce_mixed fused neighbour search + hydro force (replaces
+//                             the FDPS tree walk and dispatch/retrieve),
+//                             ending with gravity + finalKick + density floor
+//                             + the block's dt bound; the FP64 path runs
+//                             k_neigh (lists) + k_force64 + k_final instead
 // The fused force passes add their per-block pair counts into the step
-// scalars with one integer atomic per block (order-independent); k_final's
+// scalars with one integer atomic per block (order-independent); the
 // per-block dt bounds are reduced in index order by the next step's
 // k_scalars. Nothing returns to the host between steps; dt lives on the
 // device.
@@ -52,12 +53,14 @@ struct wcsph_sim {
     uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr,
              *part = nullptr;
     uint32_t *nbr = nullptr, *cnt = nullptr;
+    uint2* crange = nullptr;  // cell slot range of each slot (counting sort)
     Scalars* sc = nullptr;
     Partials P{};
     double* diag_part = nullptr;
     double* diag_out = nullptr;
     bool uploaded = false, forced = false;
     bool steppable = false;  // acc, the dt bounds and the clock are those of init or a step
+    bool fuse_final = false;  // the next fused force pass also runs the final stage of a step
     bool timing = false;
     bool use_graphs = true;
     int64_t hstep = 0;         // steps queued since init (the Shepard filter schedule)
@@ -99,7 +102,7 @@ void check_params(const wcsph_params* p) {
 
 int64_t sim_bytes(int D, int64_t n, int K, int64_t ncell_guess) {
     const int64_t master = (3 * D * 8 + 3 * 8 + 1 + 4) * n;  // pos vel vh, dens mass h, type, id
-    const int64_t per = 2 * master + (D * 8 + 16) * n + 52 * n + 4 * 3 * n + (int64_t)K * 4 * n + 4 * n;
+    const int64_t per = 2 * master + (D * 8 + 16) * n + 52 * n + 4 * 3 * n + 8 * n + (int64_t)K * 4 * n + 4 * n;
     return per + 8 * (ncell_guess + 1) + (1 << 20);
 }
 
@@ -187,40 +190,61 @@ static void setup_precision(wcsph_sim* s) {
     s->mc.half_Ls = (float)(0.5 * s->grid.L / hh);
 }
 
-static void bin_and_reorder(wcsph_sim* s) {
+// Cell list by counting sort and the gather into cell order. step: the
+// first half of a step (kick and drift) runs inside the binning pass, and
+// the reorder forms the predicted velocities.
+static void bin_and_reorder(wcsph_sim* s, bool step) {
     const int64_t n = s->n;
     Master& src = s->m[s->cur];
     Master& dst = s->m[1 - s->cur];
+    const int kick = step ? 1 : 0;
     if (s->D == 3)
-        hipLaunchKernelGGL(k_bin<3>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src.pos, n, s->grid, s->key, s->slot,
-                           s->count, s->sc);
+        hipLaunchKernelGGL(k_bin<3>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src, s->acc, s->divv, kick, n, s->grid,
+                           s->key, s->slot, s->count, s->sc);
     else
-        hipLaunchKernelGGL(k_bin<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src.pos, n, s->grid, s->key, s->slot,
-                           s->count, s->sc);
+        hipLaunchKernelGGL(k_bin<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src, s->acc, s->divv, kick, n, s->grid,
+                           s->key, s->slot, s->count, s->sc);
     hipLaunchKernelGGL(k_scan_reduce, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part);
     hipLaunchKernelGGL(k_scan_apply, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part, s->start);
-    hipLaunchKernelGGL(k_scatter, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->key, s->slot, s->start, s->perm, n);
-    hipLaunchKernelGGL(k_cell_sort, dim3(s->nb(s->ncell)), dim3(kBlock), 0, s->st, s->start, s->perm, s->ncell);
+    hipLaunchKernelGGL(k_scatter, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->key, s->slot, s->start, s->perm,
+                       s->crange, n);
     const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
     EosConsts e{s->p.rho0, B, s->p.gamma};
     const bool mixed = s->cfg.precision == WCSPH_PREC_MIXED;
+    const int keep_vel = s->cfg.precision == WCSPH_PREC_FP64 ? 1 : 0;
     const float vscale = mixed ? (float)(1.0 / s->p.c0) : 1.0f;
+    const dim3 grid(s->nb(n)), block(kBlock);
     if (s->D == 3) {
         if (mixed)
-            hipLaunchKernelGGL((k_reorder<3, MixedExtra<3>>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key,
-                               src, dst, s->pk, s->grid, s->pr64, s->c64, e, vscale, MixedExtra<3>{s->mx, s->mc}, n);
+            hipLaunchKernelGGL((k_reorder<3, MixedExtra<3>>), grid, block, 0, s->st, s->perm, s->key, s->crange, src,
+                               dst, s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale,
+                               MixedExtra<3>{s->mx, s->mc}, n);
         else
-            hipLaunchKernelGGL((k_reorder<3, NoExtra>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key, src,
-                               dst, s->pk, s->grid, s->pr64, s->c64, e, vscale, NoExtra{}, n);
+            hipLaunchKernelGGL((k_reorder<3, NoExtra>), grid, block, 0, s->st, s->perm, s->key, s->crange, src, dst,
+                               s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale, NoExtra{},
+                               n);
     } else {
         if (mixed)
-            hipLaunchKernelGGL((k_reorder<2, MixedExtra<2>>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key,
-                               src, dst, s->pk, s->grid, s->pr64, s->c64, e, vscale, MixedExtra<2>{s->mx, s->mc}, n);
+            hipLaunchKernelGGL((k_reorder<2, MixedExtra<2>>), grid, block, 0, s->st, s->perm, s->key, s->crange, src,
+                               dst, s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale,
+                               MixedExtra<2>{s->mx, s->mc}, n);
         else
-            hipLaunchKernelGGL((k_reorder<2, NoExtra>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->perm, s->key, src,
-                               dst, s->pk, s->grid, s->pr64, s->c64, e, vscale, NoExtra{}, n);
+            hipLaunchKernelGGL((k_reorder<2, NoExtra>), grid, block, 0, s->st, s->perm, s->key, s->crange, src, dst,
+                               s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale, NoExtra{},
+                               n);
     }
     s->cur = 1 - s->cur;
+}
+
+// Final-stage arguments for the current state.
+static FinalArgs final_args(const wcsph_sim* s, int kick) {
+    FinalArgs f;
+    f.m = s->m[s->cur];
+    f.g = s->p.g;
+    f.nu = s->p.nu;
+    f.rho_floor = s->p.rho_floor;
+    f.kick = kick;
+    return f;
 }
 
 template <class T>
@@ -295,6 +319,8 @@ static void launch_nforce(wcsph_sim* s) {
     a.dti = s->dti;
     a.P = s->P;
     a.s = s->sc;
+    a.fused_final = s->fuse_final ? 1 : 0;
+    a.fin = final_args(s, 1);
     hipLaunchKernelGGL((k_nforce<D, R, Q, UH, VS>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
 }
 
@@ -336,6 +362,8 @@ static void launch_nforce_mixed(wcsph_sim* s) {
     a.dti = s->dti;
     a.P = s->P;
     a.s = s->sc;
+    a.fused_final = s->fuse_final ? 1 : 0;
+    a.fin = final_args(s, 1);
     hipLaunchKernelGGL((k_nforce_mixed<D, R, VS, BT>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
 }
 
@@ -380,13 +408,13 @@ static void neigh_force(wcsph_sim* s, hipEvent_t e_before, hipEvent_t e_mid, hip
 }
 
 static void final_stage(wcsph_sim* s, int kick) {
-    const Master& m = s->m[s->cur];
+    const FinalArgs f = final_args(s, kick);
     if (s->D == 3)
-        hipLaunchKernelGGL(k_final<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->dti, s->p.g,
-                           s->p.nu, s->p.rho_floor, kick, s->sc, s->P, s->n);
+        hipLaunchKernelGGL(k_final<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, f, s->acc, s->dti, s->sc, s->P,
+                           s->n);
     else
-        hipLaunchKernelGGL(k_final<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->dti, s->p.g,
-                           s->p.nu, s->p.rho_floor, kick, s->sc, s->P, s->n);
+        hipLaunchKernelGGL(k_final<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, f, s->acc, s->dti, s->sc, s->P,
+                           s->n);
 }
 
 static hipEvent_t* step_events(wcsph_sim* s) {
@@ -444,18 +472,16 @@ static void one_step(wcsph_sim* s, bool filter) {
     hipEvent_t* ev = s->timing ? step_events(s) : nullptr;
     if (ev) WCSPH_HIP(hipEventRecord(ev[0], s->st));
     hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 1);
-    const Master& m = s->m[s->cur];
-    if (s->D == 3)
-        hipLaunchKernelGGL(k_kick_drift<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->divv, s->sc,
-                           s->n);
-    else
-        hipLaunchKernelGGL(k_kick_drift<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, m, s->acc, s->divv, s->sc,
-                           s->n);
-    bin_and_reorder(s);
+    bin_and_reorder(s, true);
     if (filter) shepard_pass(s);  // after the drift, before the pressure (src/main.cpp:105-106)
+    // the fused force passes end with the final stage; the FP64 path runs it
+    // as a pass of its own
+    const bool fused = s->cfg.precision != WCSPH_PREC_FP64;
+    s->fuse_final = fused;
     if (ev) neigh_force(s, ev[1], ev[2], ev[3]);
     else neigh_force(s, nullptr, nullptr, nullptr);
-    final_stage(s, 1);
+    s->fuse_final = false;
+    if (!fused) final_stage(s, 1);
     if (ev) {
         WCSPH_HIP(hipEventRecord(ev[4], s->st));
         ++s->tsteps;
@@ -590,6 +616,7 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         s->key = s->alloc<uint32_t>(n);
         s->slot = s->alloc<uint32_t>(n);
         s->perm = s->alloc<uint32_t>(n);
+        s->crange = s->alloc<uint2>(n);
         s->sc = s->alloc<Scalars>(1);
         s->P.nblk = s->nb(n);
         s->P.pairs = s->alloc<unsigned long long>(s->P.nblk);
@@ -649,7 +676,7 @@ int wcsph_sim_init(wcsph_sim* s) {
         WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "upload a state first");
         WCSPH_HIP(hipSetDevice(s->device));
         hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 0);
-        bin_and_reorder(s);
+        bin_and_reorder(s, false);
         neigh_force(s, nullptr, nullptr, nullptr);
         if (s->p.shepard_interval > 0) shepard_pass(s);  // after the first force (src/main.cpp:75)
         final_stage(s, 0);
@@ -731,7 +758,7 @@ int wcsph_sim_force(wcsph_sim* s) {
         hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
         s->steppable = false;  // acc is hydro only and the dt bounds are stale
         WCSPH_HIP(hipEventRecord(s->ev[0], s->st));
-        bin_and_reorder(s);
+        bin_and_reorder(s, false);
         neigh_force(s, s->ev[1], s->ev[2], s->ev[3]);
         WCSPH_HIP(hipGetLastError());
     });

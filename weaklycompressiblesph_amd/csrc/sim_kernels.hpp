@@ -43,12 +43,18 @@ struct Master {
 };
 
 struct Scalars {  // device-resident step scalars
+    // the step clock, read by every block of the step kernels
     double dt, time;
     unsigned long long dtmin_bits;
     long long step;
+    unsigned long long pad_[28];
+    // per-pass counters, updated by atomics from every block of the force
+    // and binning passes: kept 256 bytes away from the clock so that reads
+    // of dt do not queue behind them in L2
     unsigned long long pairs_last, pairs_total;
     unsigned int escaped, maxcnt, overflow, pad;
 };
+static_assert(__builtin_offsetof(Scalars, pairs_last) == 256, "counters on their own cache lines");
 
 // Per-block partial results (one slot per block of the particle kernels),
 // reduced in index order by k_reduce_partials: no same-address atomics.
@@ -85,7 +91,7 @@ __device__ __forceinline__ uint32_t cell_key(const Grid& g, const int* c) {
     return D == 3 ? (uint32_t)((c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0]) : (uint32_t)(c[1] * g.nc[0] + c[0]);
 }
 
-// mode 0: full reset; 1: commit dt = the minimum of the last k_final's
+// mode 0: full reset; 1: commit dt = the minimum of the last final stage
 // per-block bounds (reduced here in index order), time += dt, step += 1;
 // 2: reset the per-pass counters only. One block of kBlock threads.
 __global__ __launch_bounds__(kBlock) void k_scalars(Scalars* s, const double* __restrict__ dtmin, int64_t nblk,
@@ -120,40 +126,39 @@ __global__ __launch_bounds__(kBlock) void k_scalars(Scalars* s, const double* __
     s->maxcnt = 0;
 }
 
-// src/class.h:146-159 (initialKick, fullDrift, predict), all in FP64.
+// Counting sort, pass 1: cell key and slot within the cell, after the
+// first half of a step when kick != 0: initialKick and fullDrift
+// (src/class.h:146-155) in FP64 -- v_half = v + dt/2 a, rho += -rho dt div v,
+// x += dt v_half -- walls (FREEZE) keep x and v. predict (v + dt a,
+// src/class.h:156-159) is formed by k_reorder from v_half, so v itself is
+// not written here. The state is in the previous pass's cell order, so
+// neighbouring lanes mostly share a key: each run of equal keys within a
+// wave takes its slots with one atomic, made by the run's first lane and
+// broadcast to the rest.
 template <int D>
-__global__ void k_kick_drift(Master m, const double* __restrict__ acc, const double* __restrict__ divv,
-                             const Scalars* s, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double dt = s->dt;
-    m.dens[i] += -m.dens[i] * dt * divv[i];
-    if (m.type[i] != 0) return;  // FREEZE: no kick, no drift
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-        const double a = acc[d * n + i];
-        const double v = m.vel[d * n + i];
-        const double vh = v + 0.5 * dt * a;
-        m.vh[d * n + i] = vh;
-        m.pos[d * n + i] += dt * vh;
-        m.vel[d * n + i] = v + dt * a;
-    }
-}
-
-// Counting sort, pass 1: cell key and slot within the cell. The state is in
-// the previous pass's cell order, so neighbouring lanes mostly share a key:
-// each run of equal keys within a wave takes its slots with one atomic, made
-// by the run's first lane and broadcast to the rest.
-template <int D>
-__global__ void k_bin(const double* __restrict__ pos, int64_t n, Grid g, uint32_t* __restrict__ key,
-                      uint32_t* __restrict__ slot, uint32_t* __restrict__ count, Scalars* s) {
+__global__ void k_bin(Master m, const double* __restrict__ acc, const double* __restrict__ divv, int kick,
+                      int64_t n, Grid g, uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
+                      uint32_t* __restrict__ count, Scalars* s) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = i < n;
     uint32_t k = 0xffffffffu;
     if (live) {
         double x[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) x[d] = pos[d * n + i];
+        for (int d = 0; d < D; ++d) x[d] = m.pos[d * n + i];
+        if (kick) {
+            const double dt = s->dt;
+            m.dens[i] += -m.dens[i] * dt * divv[i];
+            if (m.type[i] == 0) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const double vh = m.vel[d * n + i] + 0.5 * dt * acc[d * n + i];
+                    m.vh[d * n + i] = vh;
+                    x[d] += dt * vh;
+                    m.pos[d * n + i] = x[d];
+                }
+            }
+        }
         int c[3] = {0, 0, 0};
         bool esc = false;
         cell_of<D>(g, x, c, esc);
@@ -240,29 +245,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ co
     }
 }
 
-// Counting sort, pass 2: scatter old indices to their cell slots.
+// Counting sort, pass 2: scatter old indices to their cell slots, each
+// with its cell's slot range [s0, s1).
 __global__ void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
-                          const uint32_t* __restrict__ start, uint32_t* __restrict__ perm, int64_t n) {
+                          const uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
+                          uint2* __restrict__ crange, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    perm[start[key[i]] + slot[i]] = (uint32_t)i;
-}
-
-// The slot order within a cell comes from atomics; sorting each cell's
-// segment by old index makes the permutation a function of the input only.
-__global__ void k_cell_sort(const uint32_t* __restrict__ start, uint32_t* __restrict__ perm, int64_t ncell) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const uint32_t b = start[c], e = start[c + 1];
-    for (uint32_t i = b + 1; i < e; ++i) {
-        const uint32_t v = perm[i];
-        uint32_t j = i;
-        while (j > b && perm[j - 1] > v) {
-            perm[j] = perm[j - 1];
-            --j;
-        }
-        perm[j] = v;
-    }
+    const uint32_t k = key[i];
+    const uint32_t s0 = start[k], s1 = start[k + 1];
+    const uint32_t d = s0 + slot[i];
+    perm[d] = (uint32_t)i;
+    crange[d] = make_uint2(s0, s1);
 }
 
 struct EosConsts {
@@ -281,8 +275,6 @@ __device__ __forceinline__ void tait(const EosConsts& e, double rho, double& pr,
     cs = sqrt(c2);
 }
 
-// Gather the master state into cell order and derive the force inputs
-// (CalcPressure, src/force.h:3-9 with Murnaghan, src/EoS.h:78-86).
 // Cell-local coordinate along one axis: x - (lo + c L), both steps in FP64
 // without contraction, rounded once to FP32. The oracle evaluates the same
 // expression (orc_neighbours_brute_cells).
@@ -305,51 +297,105 @@ struct Packed {  // FP32 force records in cell order
     uint32_t* pcell;
 };
 
-// Gather the master state into cell order and derive the force inputs
-// (CalcPressure, src/force.h:3-9 with Murnaghan, src/EoS.h:78-86).
+// Counting sort, last pass: gather the master state into cell order and
+// derive the force inputs (CalcPressure, src/force.h:3-9 with Murnaghan,
+// src/EoS.h:78-86). k_scatter put each cell's particles into the cell's slot
+// range in atomic order; slot s0 + r takes the member with the r-th smallest
+// previous index, so the order is a function of the state alone (cells hold
+// a handful of particles).
+// step != 0 (inside a step, after k_bin with kick): the force records take
+// the predicted velocity v + dt a (predict, src/class.h:156-159) of fluid
+// particles as v_half + dt/2 a_old, and v_half moves with them; the master
+// velocity of fluid particles is left for the final kick to write, except
+// with keep_vel (the FP64 force pass reads it). Walls copy v.
+// step == 0: velocities are copied as they are.
 // pr64/c64 (P/rho^2 and c in FP64) are written only for the FP64 path.
 template <int D, class Extra>
-__global__ void k_reorder(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ key, Master src,
-                          Master dst, Packed pk, Grid g, double* __restrict__ pr64, double* __restrict__ c64,
-                          EosConsts e, float v_scale, Extra extra, int64_t n) {
+__global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__ perm,
+                                                    const uint32_t* __restrict__ key,
+                                                    const uint2* __restrict__ crange, Master src, Master dst,
+                                                    const double* __restrict__ acc_old, const Scalars* s, int step,
+                                                    int keep_vel, Packed pk, Grid g, double* __restrict__ pr64,
+                                                    double* __restrict__ c64, EosConsts e, float v_scale,
+                                                    Extra extra, int64_t n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint32_t o = perm[p];
-    const uint32_t k = key[o];
-    int c[3];
-    key_cell<D>(g, k, c);
+    const bool live = p < n;
     float xl[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
+    int c[3] = {0, 0, 0};
+    double rho = 0.0, m = 0.0, pr = 0.0, cs = 0.0;
+    if (live) {
+        uint32_t o = perm[p];
+        const uint2 cr = crange[p];
+        const uint32_t k = key[o];
+        const uint32_t cnt = cr.y - cr.x, want = (uint32_t)p - cr.x;
+        constexpr uint32_t kC = 8;  // members ranked in registers
+        if (cnt > 1 && cnt <= kC) {
+            uint32_t mem[kC];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        const double x = src.pos[d * n + o];
-        const double v = src.vel[d * n + o];
-        dst.pos[d * n + p] = x;
-        dst.vel[d * n + p] = v;
-        dst.vh[d * n + p] = src.vh[d * n + o];
-        xl[d] = cell_local(x, g.lo[d], c[d], g.L);
-        vf[d] = (float)(v * v_scale);
+            for (uint32_t t = 0; t < kC; ++t) mem[t] = t < cnt ? perm[cr.x + t] : 0xffffffffu;
+#pragma unroll
+            for (uint32_t t = 0; t < kC; ++t) {
+                uint32_t r = 0;
+#pragma unroll
+                for (uint32_t u = 0; u < kC; ++u) r += mem[u] < mem[t] ? 1u : 0u;
+                if (t < cnt && r == want) o = mem[t];
+            }
+        } else if (cnt > kC) {
+            for (uint32_t q = cr.x; q < cr.y; ++q) {
+                const uint32_t cand = perm[q];
+                uint32_t r = 0;
+                for (uint32_t q2 = cr.x; q2 < cr.y; ++q2) r += perm[q2] < cand ? 1u : 0u;
+                if (r == want) {
+                    o = cand;
+                    break;
+                }
+            }
+        }
+        key_cell<D>(g, k, c);
+        const bool moving = step && src.type[o] == 0;
+        const double hdt = step ? 0.5 * s->dt : 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const double x = src.pos[d * n + o];
+            dst.pos[d * n + p] = x;
+            xl[d] = cell_local(x, g.lo[d], c[d], g.L);
+            double v;
+            if (moving) {
+                const double vh = src.vh[d * n + o];
+                v = vh + hdt * acc_old[d * n + o];
+                dst.vh[d * n + p] = vh;
+                if (keep_vel) dst.vel[d * n + p] = v;
+            } else {
+                v = src.vel[d * n + o];
+                dst.vel[d * n + p] = v;
+            }
+            vf[d] = (float)(v * v_scale);
+        }
+        rho = src.dens[o];
+        m = src.mass[o];
+        const double h = src.h[o];
+        dst.dens[p] = rho;
+        dst.mass[p] = m;
+        dst.h[p] = h;
+        dst.type[p] = src.type[o];
+        dst.id[p] = src.id[o];
+        tait(e, rho, pr, cs);
+        pk.px[p] = make_float4(xl[0], xl[1], xl[2], (float)c[0]);
+        pk.pv[p] = make_float4(vf[0], vf[1], vf[2], (float)m);
+        pk.pq[p] = make_float4((float)rho, (float)pr, (float)cs, (float)h);
+        pk.pcell[p] = k;
+        if (pr64) {
+            pr64[p] = pr;
+            c64[p] = cs;
+        }
     }
-    const double rho = src.dens[o], m = src.mass[o], h = src.h[o];
-    dst.dens[p] = rho;
-    dst.mass[p] = m;
-    dst.h[p] = h;
-    dst.type[p] = src.type[o];
-    dst.id[p] = src.id[o];
-    double pr, cs;
-    tait(e, rho, pr, cs);
-    pk.px[p] = make_float4(xl[0], xl[1], xl[2], (float)c[0]);
-    pk.pv[p] = make_float4(vf[0], vf[1], vf[2], (float)m);
-    pk.pq[p] = make_float4((float)rho, (float)pr, (float)cs, (float)h);
-    pk.pcell[p] = k;
-    if (pr64) {
-        pr64[p] = pr;
-        c64[p] = cs;
-    }
-    extra(p, xl, c[0], vf, rho, pr, cs, m);
+    extra.store_pair(p, live, xl, c[0], vf, rho, pr, cs, m);  // every lane takes part
 }
 
 struct NoExtra {
     __device__ void operator()(int64_t, const float*, int, const float*, double, double, double, double) const {}
+    __device__ void store_pair(int64_t, bool, const float*, int, const float*, double, double, double,
+                               double) const {}
 };
 
 // Walks the (2R+1)^D cells around particle i's cell (R = cells per
@@ -504,29 +550,42 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(Partials P, Scalars*
 // Block totals of the fused force passes, added straight into the step
 // scalars with one integer atomic each per block: integer sums, maxima and
 // ors do not depend on the order the blocks arrive in.
-__device__ __forceinline__ void block_count_atomic(uint32_t cnt, Scalars* s, bool to_total) {
+__device__ __forceinline__ void block_count_atomic(uint32_t cnt, Scalars* s, bool to_total,
+                                                   double* dt_slot = nullptr, double dt_cand = 1.0e30) {
+    // one barrier for both block reductions: the pair count (sum and max,
+    // added to the scalars with one atomic each) and, with dt_slot, the
+    // minimum of the positive dt bounds (they order like their bit patterns)
     __shared__ unsigned long long sh_sum[kBlock / 64];
     __shared__ uint32_t sh_max[kBlock / 64];
+    __shared__ unsigned long long sh_dt[kBlock / 64];
     unsigned long long v = cnt;
     uint32_t mx = cnt;
+    unsigned long long b = (unsigned long long)__double_as_longlong(dt_cand);
     for (int off = 32; off > 0; off >>= 1) {
         v += __shfl_down(v, off, 64);
         const uint32_t o = __shfl_down(mx, off, 64);
         mx = o > mx ? o : mx;
+        if (dt_slot) {
+            const unsigned long long ob = __shfl_down(b, off, 64);
+            b = ob < b ? ob : b;
+        }
     }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (lane == 0) {
         sh_sum[wv] = v;
         sh_max[wv] = mx;
+        sh_dt[wv] = b;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long t = 0;
+        unsigned long long t = 0, tb = sh_dt[0];
         uint32_t m = 0;
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
             t += sh_sum[k];
             m = sh_max[k] > m ? sh_max[k] : m;
+            tb = sh_dt[k] < tb ? sh_dt[k] : tb;
         }
+        if (dt_slot) *dt_slot = __longlong_as_double((long long)tb);
         atomicAdd(&s->pairs_last, t);
         if (to_total) atomicAdd(&s->pairs_total, t);
         atomicMax(&s->maxcnt, m);
@@ -576,6 +635,62 @@ __global__ __launch_bounds__(kBlock) void k_neigh(Packed pk, const uint32_t* __r
 // Hits are queued in LDS (index | row << 27); whenever some lane's queue is
 // full the wave evaluates all queued pairs. No neighbour list reaches HBM.
 // The summation order per i is the scan order, so results are deterministic.
+// The final stage of a step for one particle: gravity along the last axis
+// (src/init/Dam2.h:104-108), finalKick v = v_half + dt/2 a of fluid
+// particles when kick != 0 (src/class.h:160-163), the density floor
+// (src/init/Dam2.h:111) and the particle's bound in getTimeStepGlobal
+// (src/integral.h:2-12: min of dt_i, 0.25 sqrt(h/|a|), 0.125 h^2/nu over
+// fluid particles). a holds the hydro acceleration on entry and the total
+// on return; the bound is returned (1e30 for walls).
+struct FinalArgs {
+    Master m;
+    double g, nu, rho_floor;
+    int kick;
+};
+
+template <int D>
+__device__ __forceinline__ double final_particle(const FinalArgs& f, const Scalars* s, int64_t i, int64_t n, double* a,
+                                                 double dti) {
+    a[D - 1] -= f.g;
+    const bool fluid = f.m.type[i] == 0;
+    const double hdt = 0.5 * s->dt;
+    double a2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        a2 += a[d] * a[d];
+        if (f.kick && fluid) f.m.vel[d * n + i] = f.m.vh[d * n + i] + hdt * a[d];
+    }
+    if (f.m.dens[i] < f.rho_floor) f.m.dens[i] = f.rho_floor;
+    double cand = 1.0e30;
+    if (fluid) {
+        const double h = f.m.h[i];
+        const double t1 = 0.25 * sqrt(h / sqrt(a2));
+        const double t2 = 0.125 * h * h / f.nu;
+        cand = dti;
+        cand = t1 < cand ? t1 : cand;
+        cand = t2 < cand ? t2 : cand;
+    }
+    return cand;
+}
+
+// Minimum over the block of positive doubles (they order like their bit
+// patterns) into *slot. Every thread of the block calls it.
+__device__ __forceinline__ void block_min_store(double cand, double* slot) {
+    unsigned long long b = (unsigned long long)__double_as_longlong(cand);
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_down(b, off, 64);
+        b = o < b ? o : b;
+    }
+    __shared__ unsigned long long sh[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = sh[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) t = sh[k] < t ? sh[k] : t;
+        *slot = __longlong_as_double((long long)t);
+    }
+}
+
 struct NFArgs {
     Packed pk;
     const uint32_t* start;
@@ -589,6 +704,8 @@ struct NFArgs {
     double* dti;
     Partials P;
     Scalars* s;
+    int fused_final;  // run the final stage (FinalArgs) in the epilogue
+    FinalArgs fin;
 };
 
 __device__ __forceinline__ float axis_gap(int d, float x, float Lf) {
@@ -761,14 +878,20 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
         }
     }
     flush();
+    double cand = 1.0e30;
     if (live) {
         const int64_t n = a.n;
+        double ad[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) a.acc[d * n + i] = (double)acc[d];
+        for (int d = 0; d < D; ++d) ad[d] = (double)acc[d];
+        const double dti = a.c_cfl * (double)I.h / ((double)I.c + (double)vsig);
         a.divv[i] = (double)divv / (double)I.rho;
-        a.dti[i] = a.c_cfl * (double)I.h / ((double)I.c + (double)vsig);
+        a.dti[i] = dti;
+        if (a.fused_final) cand = final_particle<D>(a.fin, a.s, i, n, ad, dti);
+#pragma unroll
+        for (int d = 0; d < D; ++d) a.acc[d * n + i] = ad[d];
     }
-    block_count_atomic(total, a.s, true);
+    block_count_atomic(total, a.s, true, a.fused_final ? &a.P.dtmin[blockIdx.x] : nullptr, cand);
 }
 
 // Shepard density filter (ApplyFilter, src/force.h:10-30, with the Gather
@@ -906,50 +1029,22 @@ __global__ __launch_bounds__(kBlock) void k_force64(ForceArgs64 a) {
     a.dti[i] = a.c_cfl * I.h / (I.c + vsig);               // src/force.h:56
 }
 
-// Gravity (src/init/Dam2.h:104-108), finalKick (src/class.h:160-163),
-// density floor (src/init/Dam2.h:111) and the per-particle time-step bound
-// of getTimeStepGlobal (src/integral.h:2-12), reduced to one value per block.
+// The final stage (final_particle) as a pass of its own, for the FP64 path
+// and for init, with the per-block dt bounds in P.dtmin.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_final(Master m, double* __restrict__ acc,
-                                                  const double* __restrict__ dti, double g, double nu,
-                                                  double rho_floor, int kick, const Scalars* s, Partials P,
+__global__ __launch_bounds__(kBlock) void k_final(FinalArgs f, double* __restrict__ acc,
+                                                  const double* __restrict__ dti, const Scalars* s, Partials P,
                                                   int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double cand = 1.0e30;
     if (i < n) {
-        acc[(D - 1) * n + i] -= g;
-        const bool fluid = m.type[i] == 0;
-        double a2 = 0.0;
+        double a[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const double a = acc[d * n + i];
-            a2 += a * a;
-            if (kick && fluid) m.vel[d * n + i] = m.vh[d * n + i] + 0.5 * s->dt * a;
-        }
-        if (m.dens[i] < rho_floor) m.dens[i] = rho_floor;
-        if (fluid) {
-            const double h = m.h[i];
-            const double t1 = 0.25 * sqrt(h / sqrt(a2));
-            const double t2 = 0.125 * h * h / nu;
-            cand = dti[i];
-            cand = t1 < cand ? t1 : cand;
-            cand = t2 < cand ? t2 : cand;
-        }
+        for (int d = 0; d < D; ++d) a[d] = acc[d * n + i];
+        cand = final_particle<D>(f, s, i, n, a, dti[i]);
+        acc[(D - 1) * n + i] = a[D - 1];
     }
-    // positive doubles order like their bit patterns
-    unsigned long long b = (unsigned long long)__double_as_longlong(cand);
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_down(b, off, 64);
-        b = o < b ? o : b;
-    }
-    __shared__ unsigned long long sh[kBlock / 64];
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = b;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = sh[0];
-        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) t = sh[k] < t ? sh[k] : t;
-        P.dtmin[blockIdx.x] = __longlong_as_double((long long)t);
-    }
+    block_min_store(cand, &P.dtmin[blockIdx.x]);
 }
 
 }  // namespace wcsph
