@@ -77,6 +77,26 @@ def test_force_pass_matches_oracle(O, W, dim, dx, geom, precision, tol, cps, que
     assert st["pairs_last"] == pairs
 
 
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp64", 1e-12)])
+def test_large_sparse_grid(O, W, precision, tol):
+    # a cell grid far larger than the particles (2D, box 80 m x 60 m at
+    # cells of h = 0.026 m: ~7.1M cells, ~6900 scan tiles, so the one-block
+    # tile scan runs in two chunks of 4096) gives the same pairs and forces
+    s, p, op = developed(O, W, 2, 0.02, 20)
+    s.force(op, 8)
+    with W.Simulation.from_state(p, state_dict(s), precision=precision, box_lo=[-38.0, -28.0],
+                                 box_hi=[42.0, 32.0]) as sim:
+        sim.force()
+        out = sim.download()
+        st = sim.stats()
+        _, _, nc = sim.grid()
+        pairs = int(oracle_pairs(O, sim, s).sum())
+    assert int(nc[0]) * int(nc[1]) > 4096 * 1024
+    assert st["pairs_last"] == pairs and st["escaped"] == 0
+    assert rel(out["acc"], s.a) <= tol
+    assert rel(out["div_v"], s.divv) <= tol
+
+
 def test_force_pass_with_variable_h_and_m(O, W):
     s, p, op = developed(O, W, 3, 0.05, 10, **SMALL3)
     rng = np.random.default_rng(5)

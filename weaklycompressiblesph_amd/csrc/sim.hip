@@ -205,6 +205,7 @@ static void bin_and_reorder(wcsph_sim* s, bool step) {
         hipLaunchKernelGGL(k_bin<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src, s->acc, s->divv, kick, n, s->grid,
                            s->key, s->slot, s->count, s->sc);
     hipLaunchKernelGGL(k_scan_reduce, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanTilesThreads), 0, s->st, s->part, s->npart);
     hipLaunchKernelGGL(k_scan_apply, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part, s->start);
     hipLaunchKernelGGL(k_scatter, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->key, s->slot, s->start, s->perm,
                        s->crange, n);
@@ -471,7 +472,7 @@ static bool shepard_due(const wcsph_sim* s, int64_t step) {
 static void one_step(wcsph_sim* s, bool filter) {
     hipEvent_t* ev = s->timing ? step_events(s) : nullptr;
     if (ev) WCSPH_HIP(hipEventRecord(ev[0], s->st));
-    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 1);
+    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 1);
     bin_and_reorder(s, true);
     if (filter) shepard_pass(s);  // after the drift, before the pressure (src/main.cpp:105-106)
     // the fused force passes end with the final stage; the FP64 path runs it
@@ -675,7 +676,7 @@ int wcsph_sim_init(wcsph_sim* s) {
     return guarded([&] {
         WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "upload a state first");
         WCSPH_HIP(hipSetDevice(s->device));
-        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 0);
+        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 0);
         bin_and_reorder(s, false);
         neigh_force(s, nullptr, nullptr, nullptr);
         if (s->p.shepard_interval > 0) shepard_pass(s);  // after the first force (src/main.cpp:75)
@@ -755,7 +756,7 @@ int wcsph_sim_force(wcsph_sim* s) {
     return guarded([&] {
         WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "upload a state first");
         WCSPH_HIP(hipSetDevice(s->device));
-        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
+        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
         s->steppable = false;  // acc is hydro only and the dt bounds are stale
         WCSPH_HIP(hipEventRecord(s->ev[0], s->st));
         bin_and_reorder(s, false);
@@ -871,7 +872,7 @@ int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t 
         WCSPH_REQUIRE(!lists || cap > 0, WCSPH_EINVAL, "cap must be positive with lists");
         if (s->cfg.precision != WCSPH_PREC_FP64) {
             // the fused pass keeps no lists: rebuild them for the current cell order
-            hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kBlock), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
+            hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
             neigh_lists(s, 0);
             WCSPH_HIP(hipGetLastError());
         }

@@ -91,21 +91,36 @@ __device__ __forceinline__ uint32_t cell_key(const Grid& g, const int* c) {
     return D == 3 ? (uint32_t)((c[2] * g.nc[1] + c[1]) * g.nc[0] + c[0]) : (uint32_t)(c[1] * g.nc[0] + c[0]);
 }
 
-// mode 0: full reset; 1: commit dt = the minimum of the last final stage
-// per-block bounds (reduced here in index order), time += dt, step += 1;
-// 2: reset the per-pass counters only. One block of kBlock threads.
-__global__ __launch_bounds__(kBlock) void k_scalars(Scalars* s, const double* __restrict__ dtmin, int64_t nblk,
-                                                     int mode) {
-    __shared__ double sh[kBlock];
+// mode 0: full reset; 1: commit dt = the minimum of the last final stage's
+// per-block bounds, time += dt, step += 1; 2: reset the per-pass counters
+// only. One block of kScalarsThreads threads, four loads in flight each.
+constexpr int kScalarsThreads = 1024;
+__global__ __launch_bounds__(kScalarsThreads) void k_scalars(Scalars* s, const double* __restrict__ dtmin,
+                                                              int64_t nblk, int mode) {
+    __shared__ double sh[kScalarsThreads / 64];
     if (mode == 1) {
-        double d = 1.0e30;
-        for (int64_t b = threadIdx.x; b < nblk; b += kBlock) d = dtmin[b] < d ? dtmin[b] : d;
-        sh[threadIdx.x] = d;
-        __syncthreads();
-        for (int off = kBlock / 2; off > 0; off >>= 1) {
-            if ((int)threadIdx.x < off) sh[threadIdx.x] = sh[threadIdx.x + off] < sh[threadIdx.x] ? sh[threadIdx.x + off] : sh[threadIdx.x];
-            __syncthreads();
+        const int64_t T = kScalarsThreads;
+        double d0 = 1.0e30, d1 = 1.0e30, d2 = 1.0e30, d3 = 1.0e30;
+        int64_t b = threadIdx.x;
+        for (; b + 3 * T < nblk; b += 4 * T) {
+            const double x0 = dtmin[b], x1 = dtmin[b + T], x2 = dtmin[b + 2 * T], x3 = dtmin[b + 3 * T];
+            d0 = x0 < d0 ? x0 : d0;
+            d1 = x1 < d1 ? x1 : d1;
+            d2 = x2 < d2 ? x2 : d2;
+            d3 = x3 < d3 ? x3 : d3;
         }
+        for (; b < nblk; b += T) d0 = dtmin[b] < d0 ? dtmin[b] : d0;
+        d0 = d1 < d0 ? d1 : d0;
+        d2 = d3 < d2 ? d3 : d2;
+        double d = d2 < d0 ? d2 : d0;
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_down(d, off, 64);
+            d = o < d ? o : d;
+        }
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = d;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 1; k < kScalarsThreads / 64; ++k) sh[0] = sh[k] < sh[0] ? sh[k] : sh[0];
     }
     if (threadIdx.x != 0) return;
     if (mode == 0) {
@@ -182,28 +197,41 @@ __global__ void k_bin(Master m, const double* __restrict__ acc, const double* __
 
 // Exclusive scan of count[0..m) into start[0..m], start[m] = total, over
 // tiles of 1024 elements (256 threads x 4): k_scan_reduce writes each tile's
-// total, k_scan_apply finishes each tile.
+// total, k_scan_tiles scans the totals in place (one block), k_scan_apply
+// finishes each tile.
 constexpr int kScanTile = 1024;
 
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-        const uint32_t add = t >= off ? sh[t - off] : 0u;
-        __syncthreads();
-        sh[t] += add;
-        __syncthreads();
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)v, off, 64);
+        if (lane >= off) v += u;
     }
-    total = sh[kBlock - 1];
-    const uint32_t incl = sh[t];
+    return v;
+}
+
+// Exclusive scan over the block (a multiple of 64 threads, at most 1024);
+// total = the block's sum. sh holds at least blockDim.x / 64 words.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t incl = wave_inclusive_scan(v);
+    if (lane == 63) sh[w] = incl;
     __syncthreads();
-    return incl - v;
+    uint32_t before = 0, tot = 0;
+    for (int k = 0; k < nw; ++k) {
+        const uint32_t t = sh[k];
+        before += k < w ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();  // sh may be reused after the call
+    total = tot;
+    return before + incl - v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t m,
                                                          uint32_t* __restrict__ part) {
-    __shared__ uint32_t sh[kBlock];
+    __shared__ uint32_t sh[kBlock / 64];
     const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
     uint32_t v = 0;
 #pragma unroll
@@ -213,18 +241,37 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restri
     if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-// Second scan phase: each tile adds up the totals of the tiles before it
-// (every tile re-reads at most a few thousand words, all in L2), scans its
-// own counts, writes start and clears count for the next binning pass.
+constexpr int kScanTilesThreads = 1024;
+__global__ __launch_bounds__(kScanTilesThreads) void k_scan_tiles(uint32_t* __restrict__ part, int64_t ntiles) {
+    __shared__ uint32_t sh[kScanTilesThreads / 64];
+    uint32_t carry = 0;
+    for (int64_t b = 0; b < ntiles; b += 4 * kScanTilesThreads) {
+        const int64_t base = b + threadIdx.x * 4;
+        uint32_t v[4], sum = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            v[k] = (base + k < ntiles) ? part[base + k] : 0u;
+            sum += v[k];
+        }
+        uint32_t total;
+        uint32_t run = carry + block_exclusive_scan(sum, sh, total);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (base + k < ntiles) part[base + k] = run;
+            run += v[k];
+        }
+        carry += total;
+    }
+}
+
+// Last scan phase: each tile scans its own counts from its offset, writes
+// start and clears count for the next binning pass.
 __global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ count, int64_t m,
                                                         const uint32_t* __restrict__ part,
                                                         uint32_t* __restrict__ start) {
-    __shared__ uint32_t sh[kBlock];
+    __shared__ uint32_t sh[kBlock / 64];
     const int64_t tile = blockIdx.x;
-    uint32_t pre = 0;
-    for (int64_t t = threadIdx.x; t < tile; t += kBlock) pre += part[t];
-    uint32_t before;
-    block_exclusive_scan(pre, sh, before);
+    const uint32_t before = part[tile];
     const int64_t base = tile * kScanTile + threadIdx.x * 4;
     uint32_t v[4], sum = 0;
 #pragma unroll
