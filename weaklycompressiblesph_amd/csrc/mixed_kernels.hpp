@@ -61,30 +61,17 @@ struct MixedConsts {
     float half_Ls;  // half the cell edge in h: positions are stored relative to the cell centre
 };
 
-// Extra stores of k_reorder in mixed mode: thread p owns half p & 1 of
-// record p >> 1.
-template <int D>
-__device__ __forceinline__ void write_mixed(const Mixed& mx, const MixedConsts& mc, int64_t p, const float* xl,
-                                            int cx, const float* vf, double rho, double pr, double c, double m) {
-    const int o = (int)(p & 1);
-    _Float16* P = reinterpret_cast<_Float16*>(&mx.pos[p >> 1]) + o;
-    _Float16* V = reinterpret_cast<_Float16*>(&mx.vel[p >> 1]) + o;
-    _Float16* S = reinterpret_cast<_Float16*>(&mx.st[p >> 1]) + o;
-    P[0] = (_Float16)(xl[0] * mc.inv_h - mc.half_Ls);
-    P[2] = (_Float16)(xl[1] * mc.inv_h - mc.half_Ls);
-    P[4] = (_Float16)(D == 3 ? xl[2] * mc.inv_h - mc.half_Ls : 0.f);
-    P[6] = (_Float16)(float)cx;
-    V[0] = (_Float16)vf[0];
-    V[2] = (_Float16)vf[1];
-    V[4] = (_Float16)(D == 3 ? vf[2] : 0.f);
-    V[6] = (_Float16)(float)(m * mc.m_scale);
+// The state half of particle p's record (after the Shepard filter changes
+// the density): thread p owns half p & 1 of record p >> 1.
+__device__ __forceinline__ void write_mixed_state(const Mixed& mx, const MixedConsts& mc, int64_t p, double rho,
+                                                  double pr, double c) {
+    _Float16* S = reinterpret_cast<_Float16*>(&mx.st[p >> 1]) + (p & 1);
     S[0] = (_Float16)(float)(rho * mc.inv_rho0);
     S[2] = (_Float16)(float)(pr * mc.pr_scale);
     S[4] = (_Float16)(float)(0.5 * c * mc.inv_c0);
-    S[6] = (_Float16)0.f;
 }
 
-// The same records written whole: lanes p and p ^ 1 (one wave) swap their
+// The pair records written whole by k_reorder: lanes p and p ^ 1 (one wave) swap their
 // halves and the even lane stores the three 16-byte records. A lane past
 // the end supplies zeros. Every lane of the wave must call it.
 template <int D>
@@ -121,15 +108,15 @@ __device__ __forceinline__ void write_mixed_pair(const Mixed& mx, const MixedCon
 
 template <int D>
 struct MixedExtra {  // k_reorder / k_apply_density hook writing the pair records
+    static constexpr bool kFp32Velocities = false;  // the mixed pass reads velocities from its records
     Mixed mx;
     MixedConsts mc;
-    __device__ void operator()(int64_t p, const float* xl, int cx, const float* vf, double rho, double pr, double c,
-                               double m) const {
-        write_mixed<D>(mx, mc, p, xl, cx, vf, rho, pr, c, m);
-    }
     __device__ void store_pair(int64_t p, bool live, const float* xl, int cx, const float* vf, double rho,
                                double pr, double c, double m) const {
         write_mixed_pair<D>(mx, mc, p, live, xl, cx, vf, rho, pr, c, m);
+    }
+    __device__ void state(int64_t p, double rho, double pr, double c) const {
+        write_mixed_state(mx, mc, p, rho, pr, c);
     }
 };
 

@@ -66,10 +66,25 @@ struct Partials {
     int64_t nblk;
 };
 
-// Map blockIdx so consecutive logical blocks share an XCD (and its L2):
-// hardware places block b on XCD b % 8.
+// Map blockIdx onto particle blocks for the 8 XCDs (hardware places block b
+// on XCD b % 8, each XCD with its own L2). WCSPH_XCD_CHUNK > 0: each XCD
+// takes runs of that many consecutive particle blocks in turn, so all XCDs
+// sweep the domain together (the cost per particle varies along the cell
+// order) while neighbouring blocks still share an L2; blocks past the last
+// whole round keep their index. WCSPH_XCD_CHUNK == 0: each XCD takes one
+// contiguous eighth.
+#ifndef WCSPH_XCD_CHUNK
+#define WCSPH_XCD_CHUNK 32
+#endif
 __device__ __forceinline__ int xcd_block() {
     const int b = blockIdx.x, nb = gridDim.x;
+    constexpr int C = WCSPH_XCD_CHUNK;
+    if (C > 0) {
+        const int full = nb / (8 * C) * (8 * C);
+        if (b >= full) return b;
+        const int x = b % 8, k = b / 8;
+        return ((k / C) * 8 + x) * C + k % C;
+    }
     const int q = nb / 8, r = nb % 8, x = b % 8;
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + b / 8;
 }
@@ -428,7 +443,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
         dst.id[p] = src.id[o];
         tait(e, rho, pr, cs);
         pk.px[p] = make_float4(xl[0], xl[1], xl[2], (float)c[0]);
-        pk.pv[p] = make_float4(vf[0], vf[1], vf[2], (float)m);
+        if (Extra::kFp32Velocities) pk.pv[p] = make_float4(vf[0], vf[1], vf[2], (float)m);
         pk.pq[p] = make_float4((float)rho, (float)pr, (float)cs, (float)h);
         pk.pcell[p] = k;
         if (pr64) {
@@ -439,10 +454,11 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
     extra.store_pair(p, live, xl, c[0], vf, rho, pr, cs, m);  // every lane takes part
 }
 
-struct NoExtra {
-    __device__ void operator()(int64_t, const float*, int, const float*, double, double, double, double) const {}
+struct NoExtra {  // FP32 and FP64 paths: the FP32 records are all there is
+    static constexpr bool kFp32Velocities = true;
     __device__ void store_pair(int64_t, bool, const float*, int, const float*, double, double, double,
                                double) const {}
+    __device__ void state(int64_t, double, double, double) const {}
 };
 
 // Walks the (2R+1)^D cells around particle i's cell (R = cells per
@@ -994,7 +1010,7 @@ __global__ __launch_bounds__(kBlock) void k_shepard(Master m, Packed pk, const u
 
 // Commit the filtered density (copyFromForce, src/class.h:116-118) and redo
 // what k_reorder derived from it: P/rho^2 and c in the force records (and
-// the FP64 copies and mixed records when present).
+// the FP64 copies and the mixed state records when present).
 template <int D, class Extra>
 __global__ void k_apply_density(Master m, Packed pk, const double* __restrict__ rho_new, double* __restrict__ pr64,
                                 double* __restrict__ c64, EosConsts e, Extra extra, int64_t n) {
@@ -1010,9 +1026,7 @@ __global__ void k_apply_density(Master m, Packed pk, const double* __restrict__ 
         pr64[p] = pr;
         c64[p] = cs;
     }
-    const float4 x = pk.px[p], v = pk.pv[p];
-    const float xl[3] = {x.x, x.y, x.z}, vf[3] = {v.x, v.y, v.z};
-    extra(p, xl, (int)x.w, vf, rho, pr, cs, m.mass[p]);
+    extra.state(p, rho, pr, cs);
 }
 
 // FP64 force pass over neighbour lists (the GPU oracle path): pair
