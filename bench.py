@@ -9,11 +9,13 @@ time step: kick-drift, cell-list rebuild by counting sort, neighbour pass,
 force pass, gravity/final kick and the global dt reduction, all in HBM.
 
 value = neighbour pair interactions evaluated in the timed steps / wall time
-of those steps, for the mixed-precision path (packed FP16 pair arithmetic,
-FP32 accumulation, FP64 integration); the FP32 plain path runs the same
-steps beside it ("paths", "mixed_over_fp32"). The roofline entry is for the
+of those steps (replayed from the library's two-step hipGraphs, as users
+step), for the mixed-precision path (packed FP16 pair arithmetic, FP32
+accumulation, FP64 integration); the FP32 plain path runs the same steps
+beside it ("paths", "mixed_over_fp32"). The roofline entry is for the
 headline force kernel, from HIP events recorded on the simulation's stream
-around every timed launch. The CPU baseline times the FP64 oracle's force
+around each launch of a following window of up to 100 steps launched one by
+one. The CPU baseline times the FP64 oracle's force
 pass (the reference CPU functor, src/force.h:31-60, over cell-list
 candidates, OpenMP) on the same developed state, on the host's cores. The
 serving entry runs a seeded Poisson stream of 16 dam-break requests through
@@ -154,6 +156,45 @@ def serving_study(W, seed, count, rate, max_slots, chunk, precision, budget_gib,
             "results_identical_batched_vs_one_at_a_time": identical}
 
 
+def microbench(W, dim, dx, reps=100, warm=10, seed=12345):
+    """Force passes on a perturbed lattice: the dam-break lattice with seeded
+    jitter U(+-0.05 dx) on fluid positions, velocities N(0, (0.02 c0)^2)
+    and densities rho0 (1 + N(0, 1e-3^2)) on fluid particles. Each pass
+    re-bins, reorders and evaluates the hydro force (wcsph_sim_force); the
+    force kernel is timed with HIP events, warm passes first."""
+    import numpy as np
+    params = W.dam_break_params(dim)
+    st = W.dam_break_state(W.dam_break_desc(dim, dx), params)
+    rng = np.random.default_rng(seed)
+    fl = st["type"] == 0
+    nf = int(fl.sum())
+    st["pos"][:, fl] += rng.uniform(-0.05, 0.05, (dim, nf)) * dx
+    st["vel"][:, fl] = rng.normal(0.0, 0.02 * params.c0, (dim, nf))
+    st["dens"][fl] = params.rho0 * (1.0 + rng.normal(0.0, 1e-3, nf))
+    n = len(st["dens"])
+    out = {"workload": f"{dim}D dam-break lattice dx={dx} m, {n} particles, jittered (seed {seed}), "
+                       f"{reps} force passes after {warm}", "paths": {}}
+    for prec in ("mixed", "fp32"):
+        with W.Simulation.from_state(params, st, precision=prec) as sim:
+            for _ in range(warm):
+                sim.force()
+            f_ms, b_ms = [], []
+            for _ in range(reps):
+                sim.force()
+                t = sim.last_force_ms()
+                f_ms.append(t["force"])
+                b_ms.append(t["binning"])
+            pairs = sim.stats()["pairs_last"]
+        f = float(np.mean(f_ms))
+        out["paths"][prec] = {"pairs_per_pass": pairs, "force_ms": f, "force_ms_min": float(np.min(f_ms)),
+                              "binning_reorder_ms": float(np.mean(b_ms)),
+                              "force_kernel_pairs_per_s": pairs / (f * 1e-3),
+                              "achieved_tflops": FLOP_PER_PAIR[dim] * pairs / (f * 1e-3) / 1e12}
+    out["mixed_over_fp32"] = out["paths"]["mixed"]["force_kernel_pairs_per_s"] / \
+        out["paths"]["fp32"]["force_kernel_pairs_per_s"]
+    return out
+
+
 def fdps_study(W, dim, dx, warmup, group_edge=5.0):
     """The FDPS drop-in (GPU_WC::HydrDispatchKernel / HydrRetrieveKernel)
     end to end on a developed state: host records in, host results out,
@@ -269,6 +310,8 @@ def main():
     ap.add_argument("--t-end", type=float, default=1.0)
     ap.add_argument("--fdps", action="store_true",
                     help="time the FDPS drop-in end to end (host records, PCIe included) on a developed state")
+    ap.add_argument("--microbench", action="store_true",
+                    help="force passes on the jittered lattice (seed 12345), mixed and FP32")
     ap.add_argument("--serve", action="store_true",
                     help="run a seeded request stream through the serving loop (p50/p99 latency, makespan)")
     ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving run (0 skips it)")
@@ -287,6 +330,9 @@ def main():
     dx = args.dx or (0.0126 if dim == 3 else 0.02)
     if args.fdps:
         print(json.dumps(fdps_study(W, dim, dx, args.warmup)), flush=True)
+        return
+    if args.microbench:
+        print(json.dumps(microbench(W, dim, dx)), flush=True)
         return
     if args.serve:
         print(json.dumps(serving_study(W, args.serve_seed, args.serve_requests, args.serve_rate, args.serve_slots,
