@@ -152,3 +152,31 @@ def test_reference_style_resume_follows_oracle(O, W, tmp_path):
     assert abs(gs["time"] - s.time) <= 1e-12 * s.time
     assert np.allclose(got["dens"], s.dens, rtol=1e-11)
     assert rel(got["pos"] - state["pos"], s.x - state["pos"]) < 1e-9
+
+
+def test_output_schedule_follows_reference(W, tmp_path):
+    # OutputFileWithTimeInterval: a snapshot at the start and after each step
+    # whose time reaches the next output time; the times come from a run
+    # stepped one step at a time and read back after every step
+    p = W.dam_break_params(2)
+    st = W.dam_break_state(W.dam_break_desc(2, 0.05), p)
+    end, nsnap = 0.02, 7
+    times = [0.0]
+    with W.Simulation.from_state(p, st, precision="fp32") as sim:
+        sim.init()
+        while times[-1] < end:
+            sim.step(1)
+            times.append(sim.stats()["time"])
+    want, t_next = [], 0.0
+    for t in times:
+        if t >= t_next:
+            want.append(t)
+            t_next += end / nsnap
+    with W.Simulation.from_state(p, st, precision="fp32") as sim:
+        sim.init()
+        k = sim.run_with_output(end, nsnap, str(tmp_path))
+        assert sim.stats()["time"] == times[-1]
+    assert k == len(want)
+    got = [W.read_snapshot_ascii(W.snapshot_path(str(tmp_path), i), 2)["time"] for i in range(k)]
+    assert got == [float("%e" % t) for t in want]
+    assert (tmp_path / "00000_00001_00000.bin").exists()

@@ -288,40 +288,32 @@ class Simulation:
     def run_with_output(self, end_time, n_snapshots, outdir, *, restart=True, nproc=1, rank=0):
         """Step to end_time writing snapshots on the reference's schedule
         (OutputFileWithTimeInterval, src/io.h:2-37, called at
-        src/main.cpp:79,130): a snapshot after init and after every step that
+        src/main.cpp:79,130): a snapshot now and after every step that
         reaches the next output time, which then advances by
-        end_time / n_snapshots. Snapshot k goes to snapshot_path(outdir, k)
+        end_time / n_snapshots; stepping stops at the first step that reaches
+        end_time (src/main.cpp:89). Snapshot k goes to snapshot_path(outdir, k)
         and, with restart=True, a restart file to restart_path(outdir, k),
-        rotating over 10 like the reference's binary dumps. Steps are queued in
-        chunks of half the estimated steps left to the next output, then one
-        at a time. Returns the number of snapshots written."""
+        rotating over 10 like the reference's binary dumps. Steps are queued
+        in chunks of half the estimated steps left to the next output time
+        (the clock is read back after each chunk), then one at a time.
+        Returns the number of snapshots written."""
         os.makedirs(outdir, exist_ok=True)
-        st = self.stats()
-        t_next, k = st["time"], 0
-
-        def output():
-            nonlocal t_next, k
-            self.write_ascii(snapshot_path(outdir, k, nproc, rank))
-            if restart:
-                self.save(restart_path(outdir, k, nproc, rank))
-            t_next += end_time / n_snapshots
-            k += 1
-
-        if st["time"] >= t_next:
-            output()
+        interval = end_time / n_snapshots
+        t_next = self.stats()["time"]
+        k = 0
         while True:
             st = self.stats()
+            if st["time"] >= t_next:
+                self.write_ascii(snapshot_path(outdir, k, nproc, rank))
+                if restart:
+                    self.save(restart_path(outdir, k, nproc, rank))
+                t_next += interval
+                k += 1
             if st["time"] >= end_time:
                 return k
-            dt = st["dt"] if st["dt"] > 0 else None
             left = min(t_next, end_time) - st["time"]
-            chunk = int(0.5 * left / dt) if dt else 0
+            chunk = int(0.5 * left / st["dt"]) if st["dt"] > 0 else 0
             self.step(max(chunk, 1))
-            if chunk <= 1:
-                st = self.stats()
-                if st["time"] >= t_next:
-                    output()
-
 
 # ----------------------------------------------------------------------------
 # snapshot files
