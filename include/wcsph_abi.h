@@ -99,8 +99,15 @@ int wcsph_hydro_dispatch_2d(int32_t tag, int32_t n_walk, const wcsph_epi_hydro_2
  * batch and assigns acc, dt and div_v of every i of every walk. */
 int wcsph_hydro_retrieve_3d(int32_t tag, int32_t n_walk, const int32_t* ni, wcsph_result_hydro_3d** force);
 int wcsph_hydro_retrieve_2d(int32_t tag, int32_t n_walk, const int32_t* ni, wcsph_result_hydro_2d** force);
-/* Device time (ms) of the last force kernel of a tag, from HIP events. */
+/* Device time (ms) of the last batch's force kernels of a tag, from HIP
+ * events. */
 int wcsph_hydro_last_kernel_ms(int32_t tag, float* ms);
+/* Timing of the last batch of a tag: out[0] host packing (ms, wall clock,
+ * all chunks), out[1] device span from the first upload to the end of the
+ * result copy (ms, events), out[2] force kernels (ms, events), out[3] host
+ * unpacking in retrieve (ms), out[4] chunks the batch was cut into. Waits
+ * for the batch's device work. */
+int wcsph_hydro_last_timing(int32_t tag, double* out);
 
 /* ---------------- 2. device-resident simulation ---------------- */
 #define WCSPH_PREC_FP32 0   /* FP32 pair arithmetic and accumulation */

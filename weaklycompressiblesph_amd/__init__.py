@@ -98,6 +98,14 @@ def last_kernel_ms(tag=0):
     return ms.value
 
 
+def last_timing(tag=0):
+    """Timing of the tag's last batch: host packing, device span (uploads,
+    kernels, result copy), force kernels and host unpacking, in ms."""
+    out = np.zeros(5)
+    check(lib().wcsph_hydro_last_timing(tag, ptr(out)))
+    return dict(pack_ms=out[0], device_ms=out[1], kernel_ms=out[2], unpack_ms=out[3], chunks=int(out[4]))
+
+
 def copy_probe_gbps(nbytes=1 << 30, reps=10, device=0):
     """Measured HBM streaming-copy bandwidth (read + write GB/s)."""
     v = C.c_double()

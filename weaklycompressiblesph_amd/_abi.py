@@ -91,7 +91,7 @@ EXPORTS = [
     "wcsph_params_reference", "wcsph_last_error", "wcsph_version", "wcsph_copy_probe", "wcsph_init",
     "wcsph_finalize",
     "wcsph_hydro_dispatch_3d", "wcsph_hydro_dispatch_2d", "wcsph_hydro_retrieve_3d", "wcsph_hydro_retrieve_2d",
-    "wcsph_hydro_last_kernel_ms", "wcsph_dam_break_count", "wcsph_dam_break_fill", "wcsph_sim_bytes",
+    "wcsph_hydro_last_kernel_ms", "wcsph_hydro_last_timing", "wcsph_dam_break_count", "wcsph_dam_break_fill", "wcsph_sim_bytes",
     "wcsph_sim_create", "wcsph_sim_destroy", "wcsph_sim_upload", "wcsph_sim_init", "wcsph_sim_step",
     "wcsph_sim_force", "wcsph_sim_sync", "wcsph_sim_get_stats", "wcsph_sim_download", "wcsph_sim_neighbours",
     "wcsph_sim_diagnostics", "wcsph_sim_last_force_ms", "wcsph_sim_set_timing", "wcsph_sim_timing",
@@ -126,6 +126,7 @@ def lib():
         "wcsph_hydro_retrieve_3d": (C.c_int, [I32, I32, VP, VP]),
         "wcsph_hydro_retrieve_2d": (C.c_int, [I32, I32, VP, VP]),
         "wcsph_hydro_last_kernel_ms": (C.c_int, [I32, C.POINTER(C.c_float)]),
+        "wcsph_hydro_last_timing": (C.c_int, [I32, VP]),
         "wcsph_dam_break_count": (C.c_int, [C.POINTER(DamBreakDesc), C.POINTER(I64), C.POINTER(I64)]),
         "wcsph_dam_break_fill": (C.c_int, [C.POINTER(DamBreakDesc), C.POINTER(Params), VP, VP, VP, VP, VP, VP]),
         "wcsph_sim_bytes": (I64, [I32, I64, C.POINTER(SimConfig)]),

@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -51,24 +54,99 @@ int guarded(F&& fn) {
     }
 }
 
-// Static partition of [0, n) over up to `threads` host threads.
+// Persistent host worker threads for parallel_for: created on first use and
+// reused, since creating threads per call costs milliseconds. One job runs
+// at a time (callers on other threads wait); a job started from inside a
+// worker runs inline.
+class HostPool {
+   public:
+    static HostPool& get() {
+        static HostPool pool;
+        return pool;
+    }
+    int size() const { return (int)workers_.size() + 1; }
+    static bool in_worker() { return worker_flag(); }
+    // Runs task(k) for k in [0, parts), part 0 on the calling thread.
+    void run(int parts, const std::function<void(int)>& task) {
+        std::lock_guard<std::mutex> job(job_mu_);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            task_ = &task;
+            parts_ = parts;
+            remaining_ = parts - 1;
+            ++gen_;
+        }
+        cv_start_.notify_all();
+        task(0);
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_done_.wait(lk, [&] { return remaining_ == 0; });
+        task_ = nullptr;
+    }
+
+   private:
+    HostPool() {
+        const int n = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+        for (int w = 1; w < n; ++w) workers_.emplace_back([this, w] { loop(w); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_start_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    static bool& worker_flag() {
+        static thread_local bool f = false;
+        return f;
+    }
+    void loop(int w) {
+        worker_flag() = true;
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)>* task;
+            int parts;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_start_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                task = task_;
+                parts = parts_;
+            }
+            if (w < parts) {
+                (*task)(w);
+                std::lock_guard<std::mutex> lk(mu_);
+                if (--remaining_ == 0) cv_done_.notify_one();
+            }
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex job_mu_, mu_;
+    std::condition_variable cv_start_, cv_done_;
+    const std::function<void(int)>* task_ = nullptr;
+    int parts_ = 0, remaining_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// Static partition of [0, n) over up to `threads` host threads (0: all of
+// the pool's), on the persistent pool; fewer than min_n items run inline.
 template <class F>
-void parallel_for(int64_t n, F&& fn, int threads = 0) {
-    if (threads <= 0) threads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-    if (n < 4096 || threads == 1) {
+void parallel_for(int64_t n, F&& fn, int threads = 0, int64_t min_n = 4096) {
+    HostPool& pool = HostPool::get();
+    if (threads <= 0 || threads > pool.size()) threads = pool.size();
+    if (n < min_n || threads == 1 || HostPool::in_worker()) {
         for (int64_t i = 0; i < n; ++i) fn(i);
         return;
     }
-    std::vector<std::thread> pool;
-    const int64_t chunk = (n + threads - 1) / threads;
-    for (int t = 0; t < threads; ++t) {
-        const int64_t b = t * chunk, e = std::min(n, b + chunk);
-        if (b >= e) break;
-        pool.emplace_back([&fn, b, e] {
-            for (int64_t i = b; i < e; ++i) fn(i);
-        });
-    }
-    for (auto& th : pool) th.join();
+    const int parts = (int)std::min<int64_t>(threads, n);
+    const int64_t chunk = (n + parts - 1) / parts;
+    const std::function<void(int)> task = [&](int k) {
+        const int64_t b = k * chunk, e = std::min(n, b + chunk);
+        for (int64_t i = b; i < e; ++i) fn(i);
+    };
+    pool.run(parts, task);
 }
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
