@@ -1,8 +1,8 @@
 """ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg import this module, as the checker or the timed CPU
-baseline. The product package never imports it.
+cpu_baseline leg import this module, as the reference results are
+compared with or as the timed CPU baseline. The product package never imports it.
 
 The oracle restates the reference's hydro force path:
   src/force.h:31-60  CalcHydroForce  -> hydro_force / hydro_force_walks

@@ -2,8 +2,8 @@
  * wcsph_oracle.h -- FP64 CPU restatement of the reference hydro force path.
  *
  * TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load this library, and only as the checker (or as the
- * timed CPU baseline). The product library never links or calls it.
+ * cpu_baseline leg may load this library, and only as the reference results
+ * are compared with (or as the timed CPU baseline). The product library never links or calls it.
  *
  * Each function cites the reference file:line it restates. The reference's
  * own build (FDPS-based) is not buildable in this image, so every function

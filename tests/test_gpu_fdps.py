@@ -3,7 +3,7 @@ against the FP64 oracle's restatement of src/force.h:31-60.
 
 Walks come from the oracle's fake FDPS driver (Morton-ordered i-groups of
 at most 128, superset j lists, shuffled), batched and called in FDPS's order
-(retrieve(k-1) before dispatch(k)). Tolerances (north star): normwise
+(retrieve(k-1) before dispatch(k)). Tolerances: normwise
 relative error <= 1e-5 on acc and div_v (drho/dt = -rho div_v); dt to 1e-6.
 """
 import os

@@ -71,7 +71,7 @@ struct Partials {
 // takes runs of that many consecutive particle blocks in turn, so all XCDs
 // sweep the domain together (the cost per particle varies along the cell
 // order) while neighbouring blocks still share an L2; blocks past the last
-// whole round keep their index. WCSPH_XCD_CHUNK == 0: each XCD takes one
+// whole cycle of 8 runs keep their index. WCSPH_XCD_CHUNK == 0: each XCD takes one
 // contiguous eighth.
 #ifndef WCSPH_XCD_CHUNK
 #define WCSPH_XCD_CHUNK 32
