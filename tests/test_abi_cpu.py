@@ -132,3 +132,10 @@ def test_fdps_bridge_header_compiles(two_d, tmp_path):
 def test_bridge_driver_binaries_built():
     for d in ("2d", "3d"):
         assert os.path.exists(os.path.join(ROOT, "tests", "cpp", f"bridge_driver_{d}"))
+    assert os.path.exists(os.path.join(ROOT, "tests", "cpp", "sim_driver"))
+
+
+def test_sim_driver_compiles_against_the_c_header():
+    # the device-resident driver uses nothing but include/wcsph_abi.h
+    subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "sim_driver.cpp")], check=True)

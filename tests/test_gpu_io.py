@@ -180,3 +180,33 @@ def test_output_schedule_follows_reference(W, tmp_path):
     got = [W.read_snapshot_ascii(W.snapshot_path(str(tmp_path), i), 2)["time"] for i in range(k)]
     assert got == [float("%e" % t) for t in want]
     assert (tmp_path / "00000_00001_00000.bin").exists()
+
+
+def test_cpp_driver_matches_python_and_resumes(W, tmp_path):
+    # the C++ program of tests/cpp/sim_driver.cpp (C-ABI only) writes the same
+    # snapshot files as Simulation.run_with_output, and a run resumed from
+    # its restart file of snapshot 3 writes the same later snapshots
+    import os
+    import subprocess
+    drv = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "sim_driver")
+    end, nsnap, k = 0.02, 6, 5
+    p = W.dam_break_params(2, shepard_interval=k)
+    st = W.dam_break_state(W.dam_break_desc(2, 0.05), p)
+    dpy, d1, d2 = (tmp_path / x for x in ("py", "cpp", "resumed"))
+    for d in (dpy, d1, d2):
+        d.mkdir()
+    with W.Simulation.from_state(p, st, precision="fp32") as sim:
+        sim.init()
+        n = sim.run_with_output(end, nsnap, str(dpy))
+    args = ["2", "0.05", repr(end), str(nsnap)]
+    subprocess.run([drv, *args, str(d1), "0", str(k)], check=True, timeout=120)
+    subprocess.run([drv, *args, str(d2), "0", str(k), str(d1 / "00003_00001_00000.bin"), "3"], check=True,
+                   timeout=120)
+    names = sorted(f.name for f in dpy.glob("*.dat"))
+    assert len(names) == n and names == sorted(f.name for f in d1.glob("*.dat"))
+    for f in names:
+        assert (d1 / f).read_bytes() == (dpy / f).read_bytes(), f
+    resumed = sorted(f.name for f in d2.glob("*.dat"))
+    assert resumed == names[3:]
+    for f in resumed:
+        assert (d2 / f).read_bytes() == (d1 / f).read_bytes(), f
