@@ -139,3 +139,14 @@ def test_sim_driver_compiles_against_the_c_header():
     # the device-resident driver uses nothing but include/wcsph_abi.h
     subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "cpp", "sim_driver.cpp")], check=True)
+
+
+def test_host_worker_pool(tmp_path):
+    # parallel_for's persistent pool: each index once, nested calls inline,
+    # exceptions rethrown after every part, concurrent callers (host code only)
+    exe = str(tmp_path / "pool_check")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-std=c++17", "-O1", "-x", "hip", "--offload-arch=gfx950",
+                    "--cuda-host-only", os.path.join(ROOT, "tests", "cpp", "pool_check.cpp"), "-o", exe,
+                    "-pthread"], check=True, capture_output=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120)
+    assert "pool checks: ok" in out.stdout

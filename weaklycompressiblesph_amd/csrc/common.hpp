@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <condition_variable>
+#include <exception>
 #include <functional>
 #include <mutex>
 #include <cstdint>
@@ -57,7 +58,7 @@ int guarded(F&& fn) {
 // Persistent host worker threads for parallel_for: created on first use and
 // reused, since creating threads per call costs milliseconds. One job runs
 // at a time (callers on other threads wait); a job started from inside a
-// worker runs inline.
+// running job, on a worker or on the calling thread, runs inline.
 class HostPool {
    public:
     static HostPool& get() {
@@ -67,6 +68,7 @@ class HostPool {
     int size() const { return (int)workers_.size() + 1; }
     static bool in_worker() { return worker_flag(); }
     // Runs task(k) for k in [0, parts), part 0 on the calling thread.
+    // An exception from any part is rethrown here once every part is done.
     void run(int parts, const std::function<void(int)>& task) {
         std::lock_guard<std::mutex> job(job_mu_);
         {
@@ -74,13 +76,27 @@ class HostPool {
             task_ = &task;
             parts_ = parts;
             remaining_ = parts - 1;
+            error_ = nullptr;
             ++gen_;
         }
         cv_start_.notify_all();
-        task(0);
+        std::exception_ptr mine;
+        {  // a parallel_for nested in part 0 runs inline instead of waiting on this job
+            const bool was = worker_flag();
+            worker_flag() = true;
+            try {
+                task(0);
+            } catch (...) {
+                mine = std::current_exception();
+            }
+            worker_flag() = was;
+        }
         std::unique_lock<std::mutex> lk(mu_);
         cv_done_.wait(lk, [&] { return remaining_ == 0; });
         task_ = nullptr;
+        if (!mine) mine = error_;
+        lk.unlock();
+        if (mine) std::rethrow_exception(mine);
     }
 
    private:
@@ -115,8 +131,14 @@ class HostPool {
                 parts = parts_;
             }
             if (w < parts) {
-                (*task)(w);
+                std::exception_ptr e;
+                try {
+                    (*task)(w);
+                } catch (...) {
+                    e = std::current_exception();
+                }
                 std::lock_guard<std::mutex> lk(mu_);
+                if (e && !error_) error_ = e;
                 if (--remaining_ == 0) cv_done_.notify_one();
             }
         }
@@ -125,6 +147,7 @@ class HostPool {
     std::mutex job_mu_, mu_;
     std::condition_variable cv_start_, cv_done_;
     const std::function<void(int)>* task_ = nullptr;
+    std::exception_ptr error_;
     int parts_ = 0, remaining_ = 0;
     uint64_t gen_ = 0;
     bool stop_ = false;
