@@ -5,21 +5,27 @@
 // In these units every quantity of the pair body is O(1e-4..50) and fits
 // FP16 (in SI units 1/H^4 alone would overflow it).
 //
-// Pair records: particles 2m and 2m+1 (cell order) share one 16-byte record
-// of each kind, as the two halves of packed half2 values:
+// Pair records: particles 2m and 2m+1 (cell order) share one record of
+// each kind, the FP16 ones as the two halves of packed half2 values:
 //   PosRec16   {x, y, z, x-cell}   position relative to the cell centre / h
 //                                  (|x| <= L/2h, where FP16 has its finest
 //                                  steps) and the cell's x index
 //   VelRec16   {vx, vy, vz, m}     v / c0 and the scaled mass
-//   StateRec16 {rho, pr, c/2, -}   rho / rho0, (P / rho^2)(rho0 / c0^2), c / (2 c0)
-// One 16-byte load therefore brings two candidates, and every pair-body
-// operation is one packed FP16 VALU op (v_pk_fma_f16, v_pk_mul_f16 ...) for
+//   StateRec24 {rho[2], pr[2] (FP32), c/2 (half2)}  rho / rho0,
+//                                  (P / rho^2)(rho0 / c0^2), c / (2 c0)
+// One 16-byte load therefore brings two candidates' positions, and the
+// geometric part of the pair body -- differences, r^2, dr.dv, the spline
+// factor -- is one packed FP16 VALU op (v_pk_fma_f16, v_pk_mul_f16 ...) for
 // two pairs. Pair differences are taken in FP16 from the h-scaled cell-local
 // coordinates, the small difference first and the whole-cell offset last:
 // dx = (x_i - x_j) + (cx_i - cx_j) L/h, dy = (y_i - y_j) - dy_row L/h, so only
 // the last rounding is at the scale of the result. The scan's predicate uses
-// the cheaper per-row form (y_i - dy_row L/h) - y_j. Per-particle sums
-// accumulate in FP32 through v_dot2_f32_f16.
+// the cheaper per-row form (y_i - dy_row L/h) - y_j. Density and pressure
+// stay FP32: 1/(rho_i + rho_j) and P_i/rho_i^2 + P_j/rho_j^2 + Pi are formed
+// per pair in FP32 from the FP32 records, and so are the two pair
+// coefficients they enter (pressure and viscous). The per-particle sums
+// accumulate those FP32 coefficients times the FP16 differences in FP32
+// (v_fma_mix_f32), and div v through v_dot2_f32_f16.
 //
 // Scan: per row of cells each lane walks the records of its chord-clipped
 // window; a record with at least one neighbour half goes into the lane's
@@ -41,14 +47,17 @@ struct alignas(16) PosRec16 {
 struct alignas(16) VelRec16 {
     h2 vx, vy, vz, m;
 };
-struct alignas(16) StateRec16 {
-    h2 rho, pr, ch, pad;
+struct alignas(8) StateRec24 {
+    float rho[2], pr[2];
+    h2 ch;
+    uint32_t pad;
 };
+static_assert(sizeof(StateRec24) == 24, "state record layout");
 
 struct Mixed {
     PosRec16* pos;  // [(n + 1) / 2]
     VelRec16* vel;
-    StateRec16* st;
+    StateRec24* st;
 };
 
 __host__ __device__ inline int64_t mixed_records(int64_t n) { return (n + 1) / 2; }
@@ -61,38 +70,44 @@ struct MixedConsts {
     float half_Ls;  // half the cell edge in h: positions are stored relative to the cell centre
 };
 
-// The state half of particle p's record (after the Shepard filter changes
+// The state of particle p in its records (after the Shepard filter changes
 // the density): thread p owns half p & 1 of record p >> 1.
 __device__ __forceinline__ void write_mixed_state(const Mixed& mx, const MixedConsts& mc, int64_t p, double rho,
                                                   double pr, double c) {
-    _Float16* S = reinterpret_cast<_Float16*>(&mx.st[p >> 1]) + (p & 1);
-    S[0] = (_Float16)(float)(rho * mc.inv_rho0);
-    S[2] = (_Float16)(float)(pr * mc.pr_scale);
-    S[4] = (_Float16)(float)(0.5 * c * mc.inv_c0);
+    const int o = (int)(p & 1);
+    StateRec24& S = mx.st[p >> 1];
+    S.rho[o] = (float)(rho * mc.inv_rho0);
+    S.pr[o] = (float)(pr * mc.pr_scale);
+    reinterpret_cast<_Float16*>(&S.ch)[o] = (_Float16)(float)(0.5 * c * mc.inv_c0);
 }
 
-// The pair records written whole by k_reorder: lanes p and p ^ 1 (one wave) swap their
-// halves and the even lane stores the three 16-byte records. A lane past
+// The pair records written whole by k_reorder: lanes p and p ^ 1 (one wave)
+// swap their values and the even lane stores the three records. A lane past
 // the end supplies zeros. Every lane of the wave must call it.
 template <int D>
 __device__ __forceinline__ void write_mixed_pair(const Mixed& mx, const MixedConsts& mc, int64_t p, bool live,
                                                  const float* xl, int cx, const float* vf, double rho, double pr,
                                                  double c, double m) {
-    uint32_t mine[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    uint32_t mine[4] = {0u, 0u, 0u, 0u};
+    uint32_t fs[3] = {0u, 0u, 0u};  // rho, pr (FP32 bits) and c/2 (FP16 bits)
     if (live) {
-        const h2 v[6] = {
+        const h2 v[4] = {
             h2{(_Float16)(xl[0] * mc.inv_h - mc.half_Ls), (_Float16)(xl[1] * mc.inv_h - mc.half_Ls)},
             h2{(_Float16)(D == 3 ? xl[2] * mc.inv_h - mc.half_Ls : 0.f), (_Float16)(float)cx},
             h2{(_Float16)vf[0], (_Float16)vf[1]},
-            h2{(_Float16)(D == 3 ? vf[2] : 0.f), (_Float16)(float)(m * mc.m_scale)},
-            h2{(_Float16)(float)(rho * mc.inv_rho0), (_Float16)(float)(pr * mc.pr_scale)},
-            h2{(_Float16)(float)(0.5 * c * mc.inv_c0), (_Float16)0.f}};
+            h2{(_Float16)(D == 3 ? vf[2] : 0.f), (_Float16)(float)(m * mc.m_scale)}};
 #pragma unroll
-        for (int k = 0; k < 6; ++k) mine[k] = as_u32(v[k]);
+        for (int k = 0; k < 4; ++k) mine[k] = as_u32(v[k]);
+        fs[0] = __builtin_bit_cast(uint32_t, (float)(rho * mc.inv_rho0));
+        fs[1] = __builtin_bit_cast(uint32_t, (float)(pr * mc.pr_scale));
+        fs[2] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)(float)(0.5 * c * mc.inv_c0));
     }
-    uint32_t other[6];
+    uint32_t other[4];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) other[k] = (uint32_t)__shfl_xor((int)mine[k], 1, 64);
+    for (int k = 0; k < 4; ++k) other[k] = (uint32_t)__shfl_xor((int)mine[k], 1, 64);
+    uint32_t ofs[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ofs[k] = (uint32_t)__shfl_xor((int)fs[k], 1, 64);
     if ((p & 1) == 0 && live) {
         // record half 0 = this lane (even p), half 1 = the partner
         auto lo = [](uint32_t a, uint32_t b) { return h2{as_h2(a).x, as_h2(b).x}; };
@@ -101,8 +116,14 @@ __device__ __forceinline__ void write_mixed_pair(const Mixed& mx, const MixedCon
                                   hi(mine[1], other[1])};
         mx.vel[p >> 1] = VelRec16{lo(mine[2], other[2]), hi(mine[2], other[2]), lo(mine[3], other[3]),
                                   hi(mine[3], other[3])};
-        mx.st[p >> 1] = StateRec16{lo(mine[4], other[4]), hi(mine[4], other[4]), lo(mine[5], other[5]),
-                                   hi(mine[5], other[5])};
+        StateRec24 S;
+        S.rho[0] = __builtin_bit_cast(float, fs[0]);
+        S.rho[1] = __builtin_bit_cast(float, ofs[0]);
+        S.pr[0] = __builtin_bit_cast(float, fs[1]);
+        S.pr[1] = __builtin_bit_cast(float, ofs[1]);
+        S.ch = as_h2(fs[2] | (ofs[2] << 16));
+        S.pad = 0u;
+        mx.st[p >> 1] = S;
     }
 }
 
@@ -144,8 +165,9 @@ struct NFMixedArgs {
     int64_t n;
     int64_t nrec;  // pair records
     float h;
-    // wave-uniform packed constants (kept in scalar registers)
-    h2 k_eps, k_malpha, k_beta, k_Ls;
+    // wave-uniform constants (kept in scalar registers)
+    h2 k_eps, k_Ls;
+    float malpha, beta;  // -2 alpha, 2 beta
     double c0, nu, c_cfl, inv_rho0;
     double* acc;
     double* divv;
@@ -216,15 +238,18 @@ void k_nforce_mixed(NFMixedArgs a) {
     const bool odd = (ii & 1) != 0;
     const PosRec16 Pi = a.mx.pos[ii >> 1];
     const VelRec16 Vi = a.mx.vel[ii >> 1];
-    const StateRec16 Si = a.mx.st[ii >> 1];
+    const StateRec24 Si = a.mx.st[ii >> 1];
     auto pick = [&](h2 v) {
         const _Float16 s = odd ? v.y : v.x;
         return h2{s, s};
     };
     const h2 xi2 = pick(Pi.x), yi2 = pick(Pi.y), zi2 = pick(Pi.z), cxi2 = pick(Pi.cx);
     const h2 vxi = pick(Vi.vx), vyi = pick(Vi.vy), vzi = pick(Vi.vz);
-    const h2 rhoi = pick(Si.rho), pri = pick(Si.pr), chi = pick(Si.ch);
-    const h2 Ls2 = a.k_Ls, eps2 = a.k_eps, malpha2 = a.k_malpha, beta2 = a.k_beta;
+    const float rhoi = odd ? Si.rho[1] : Si.rho[0], pri = odd ? Si.pr[1] : Si.pr[0];
+    const h2 chi2 = pick(Si.ch);
+    const float beta = a.beta;
+    const h2 malpha2 = hsplat(a.malpha);
+    const h2 Ls2 = a.k_Ls, eps2 = a.k_eps;
     const h2 zero2 = hsplat(0.f), tiny2 = hsplat(1.0e-6f), sixteen2 = hsplat(16.f);
     const _Float16 four = (_Float16)4.f;
     const float Lf = a.g.Lf, invLf = 1.0f / Lf;
@@ -238,15 +263,16 @@ void k_nforce_mixed(NFMixedArgs a) {
 
     // Two pair interactions (the halves of record m), each kept only if its
     // hit bit is set: the other half is then outside the window, beyond the
-    // support or the particle itself. Every lane runs this for every drain
-    // step; a lane with nothing left passes e = 0 (no hit bits), which adds
-    // exactly zero.
+    // support or the particle itself. The geometry is packed FP16; the
+    // state terms and the coefficients are FP32 per half. Every lane runs
+    // this for every drain step; a lane with nothing left passes e = 0 (no
+    // hit bits), which adds exactly zero.
     auto body = [&](uint32_t e) {
         const uint32_t m = e & kRecMask;
         const uint4 T = etab[e >> kMixRecBits];
         const PosRec16 P = a.mx.pos[m];
         const VelRec16 V = a.mx.vel[m];
-        const StateRec16 St = a.mx.st[m];
+        const StateRec24 St = a.mx.st[m];
         total += T.w;
         const h2 dx = hfma(cxi2 - P.cx, Ls2, xi2 - P.x);
         const h2 dy = (yi2 - P.y) - as_h2(T.x);
@@ -266,30 +292,34 @@ void k_nforce_mixed(NFMixedArgs a) {
         r2 = hmin(hmax(r2, tiny2), sixteen2);
         if (VS == 1) drdv = as_h2(as_u32(drdv) & T.z);
         const h2 mj = as_h2(as_u32(V.m) & T.z);
-        const h2 den = r2 + eps2;              // r^2 + eps h_i h_j
-        const h2 rho_s = rhoi + St.rho;
-        const h2 X = hrcp(den * rho_s);        // 1 / (den (rho_i + rho_j))
-        const h2 inv_den = X * rho_s, inv_rs = X * den;
+        const h2 inv_den = hrcp(r2 + eps2);     // 1 / (r^2 + eps h_i h_j)
         const h2 mu = hmin(drdv, zero2) * inv_den;  // (h_i + h_j)/2 = 1
         if (VS == 1) vsig2 = hmax(vsig2, -mu);
-        const h2 cbar = chi + St.ch;
-        // Pi (rho0 / c0^2) = (-2 alpha cbar + 2 beta mu) mu / (rho_i + rho_j)
-        const h2 Pi_ = (BT ? hfma(malpha2, cbar, beta2 * mu) : malpha2 * cbar) * mu * inv_rs;
+        const h2 cbar = chi2 + St.ch;
+        const h2 amu = BT ? hfma(malpha2, cbar, hsplat(beta) * mu) * mu : malpha2 * cbar * mu;  // Pi (rho_i + rho_j)
         const h2 invr = hrsq(r2);
         const h2 rr = r2 * invr;               // r / h
         const h2 aa = hmax(hfma(rr, hsplat(-0.5f), hsplat(1.f)), zero2);
         const h2 bb = hmax(hfma(rr, hsplat(-0.5f), hsplat(0.5f)), zero2);
         const h2 g = hfma(bb * bb, hsplat(12.f), (aa * aa) * hsplat(-3.f));
         const h2 mF = mj * g * invr;           // m_j F (scaled)
-        const h2 cp = -(mF * (pri + St.pr + Pi_));
-        const h2 vc = mF * (r2 * X);
-        accp[0] = __builtin_amdgcn_fdot2(cp, dx, accp[0], false);
-        accp[1] = __builtin_amdgcn_fdot2(cp, dy, accp[1], false);
-        accv[0] = __builtin_amdgcn_fdot2(vc, dvx, accv[0], false);
-        accv[1] = __builtin_amdgcn_fdot2(vc, dvy, accv[1], false);
-        if (D == 3) {
-            accp[2] = __builtin_amdgcn_fdot2(cp, dz, accp[2], false);
-            accv[2] = __builtin_amdgcn_fdot2(vc, dvz, accv[2], false);
+        const h2 mFv = mF * (r2 * inv_den);    // m_j F r^2 / den, the viscous factor before 1/(rho_i + rho_j)
+        // density and pressure terms in FP32, one half at a time
+        const float amus[2] = {(float)amu.x, (float)amu.y};
+        const float mFs[2] = {(float)mF.x, (float)mF.y};
+        const float mFvs[2] = {(float)mFv.x, (float)mFv.y};
+        const float dxs[2][3] = {{(float)dx.x, (float)dy.x, (float)dz.x}, {(float)dx.y, (float)dy.y, (float)dz.y}};
+        const float dvs[2][3] = {{(float)dvx.x, (float)dvy.x, (float)dvz.x}, {(float)dvx.y, (float)dvy.y, (float)dvz.y}};
+#pragma unroll
+        for (int hh = 0; hh < 2; ++hh) {
+            const float inv_rs = __builtin_amdgcn_rcpf(rhoi + St.rho[hh]);  // 1 / (rho_i + rho_j)
+            const float cp = -(mFs[hh] * fmaf(amus[hh], inv_rs, pri + St.pr[hh]));
+            const float vc = mFvs[hh] * inv_rs;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                accp[d] = fmaf(cp, dxs[hh][d], accp[d]);
+                accv[d] = fmaf(vc, dvs[hh][d], accv[d]);
+            }
         }
         divv = __builtin_amdgcn_fdot2(mF, drdv, divv, false);
     };

@@ -351,8 +351,8 @@ static void launch_nforce_mixed(wcsph_sim* s) {
     a.h = s->hmax;
     auto splat = [](double v) { return h2{(_Float16)(float)v, (_Float16)(float)v}; };
     a.k_eps = splat(s->p.eps);
-    a.k_malpha = splat(-2.0 * s->p.alpha);
-    a.k_beta = splat(2.0 * s->p.beta);
+    a.malpha = (float)(-2.0 * s->p.alpha);
+    a.beta = (float)(2.0 * s->p.beta);
     a.k_Ls = splat(s->grid.L / (double)s->hmax);
     a.c0 = s->p.c0;
     a.nu = s->p.nu;
@@ -608,11 +608,11 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         if (s->cfg.precision == WCSPH_PREC_MIXED) {
             s->mx.pos = s->alloc<PosRec16>(mixed_records(n));
             s->mx.vel = s->alloc<VelRec16>(mixed_records(n));
-            s->mx.st = s->alloc<StateRec16>(mixed_records(n));
+            s->mx.st = s->alloc<StateRec24>(mixed_records(n));
             // the high half of the last record stays zero when n is odd
             WCSPH_HIP(hipMemsetAsync(s->mx.pos, 0, sizeof(PosRec16) * mixed_records(n), s->st));
             WCSPH_HIP(hipMemsetAsync(s->mx.vel, 0, sizeof(VelRec16) * mixed_records(n), s->st));
-            WCSPH_HIP(hipMemsetAsync(s->mx.st, 0, sizeof(StateRec16) * mixed_records(n), s->st));
+            WCSPH_HIP(hipMemsetAsync(s->mx.st, 0, sizeof(StateRec24) * mixed_records(n), s->st));
         }
         s->key = s->alloc<uint32_t>(n);
         s->slot = s->alloc<uint32_t>(n);
