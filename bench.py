@@ -10,9 +10,9 @@ force pass, gravity/final kick and the global dt reduction, all in HBM.
 
 value = neighbour pair interactions evaluated in the timed steps / wall time
 of those steps (replayed from the library's two-step hipGraphs, as users
-step), for the mixed-precision path (packed FP16 pair arithmetic, FP32
-accumulation, FP64 integration); the FP32 plain path runs the same steps
-beside it ("paths", "mixed_over_fp32"). The roofline entry is for the
+step), for the mixed-precision path (packed FP16 pair geometry, FP32
+density, pressure and accumulation, FP64 integration); the FP32 plain path
+runs the same steps beside it ("paths", "mixed_over_fp32"). The roofline entry is for the
 headline force kernel, from HIP events recorded on the simulation's stream
 around each launch of a following window of up to 100 steps launched one by
 one. The CPU baseline times the FP64 oracle's force

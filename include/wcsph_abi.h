@@ -111,7 +111,7 @@ int wcsph_hydro_last_timing(int32_t tag, double* out);
 
 /* ---------------- 2. device-resident simulation ---------------- */
 #define WCSPH_PREC_FP32 0   /* FP32 pair arithmetic and accumulation */
-#define WCSPH_PREC_MIXED 1  /* h-scaled FP16 pair differences, packed FP16, FP32 accumulation */
+#define WCSPH_PREC_MIXED 1  /* h-scaled FP16 pair differences on packed FP16 ops; FP32 density, pressure, accumulation */
 #define WCSPH_PREC_FP64 2   /* FP64 everywhere (the GPU oracle) */
 
 typedef struct wcsph_sim wcsph_sim;

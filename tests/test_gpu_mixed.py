@@ -1,17 +1,18 @@
-"""GPU checks of the mixed-precision path (h-scaled FP16 pair differences,
-packed FP16 pair body, FP32 accumulation, FP64 integration).
+"""GPU checks of the mixed-precision path (h-scaled FP16 pair differences
+and packed FP16 geometry; FP32 density, pressure and pair coefficients;
+FP32 accumulation; FP64 integration).
 
 The mixed path is not held to the FP32 path's 1e-5 bar: its error against
 the FP64 oracle is measured and bounded here. Tolerances: one force pass on
 a developed dam-break state, normwise relative error <= 1e-2 on acc and
-5e-3 on div_v. Measured on the 2D dx = 0.01 state: acc 5.4e-3, div_v
-1.3e-3. A numpy FP16 emulation of the same pass attributes the acc error
-to the FP16 cell-local positions (alone 1.0e-3), the FP16 per-particle
-state (1.2e-3) and the FP16 pair arithmetic (5.8e-3; no single rounding
-above 1.3e-3), amplified because the pressure gradient at depth is a sum
-of O(2(H-z)/h) larger terms that cancel. Pair count within 0.5% of the
-FP32 path's exact count (the FP16 predicate blurs the support edge by
-~1e-3 h).
+5e-3 on div_v. Measured: 2D dx = 0.01, acc 4.9e-3, div_v 1.25e-3; 3D
+dx = 0.03, acc 2.2e-3, div_v 1.7e-3. A numpy emulation of the same pass
+(each FP16 rounding applied alone) attributes the 2D acc error to the FP16
+cell-local positions (1.0e-3), the FP16 mass (1.1e-3) and the FP16 pair
+arithmetic (5.7e-3; no single rounding above 1.3e-3), amplified because
+the pressure gradient at depth is a sum of O(2(H-z)/h) larger terms that
+cancel. Pair count within 0.5% of the FP32 path's exact count (the FP16
+predicate blurs the support edge by ~1e-3 h).
 """
 import numpy as np
 import pytest

@@ -74,7 +74,7 @@ struct Partials {
 // whole cycle of 8 runs keep their index. WCSPH_XCD_CHUNK == 0: each XCD takes one
 // contiguous eighth.
 #ifndef WCSPH_XCD_CHUNK
-#define WCSPH_XCD_CHUNK 32
+#define WCSPH_XCD_CHUNK 64
 #endif
 __device__ __forceinline__ int xcd_block() {
     const int b = blockIdx.x, nb = gridDim.x;
