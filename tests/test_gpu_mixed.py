@@ -80,6 +80,40 @@ def test_mixed_steps_track_fp64(O, W):
     assert rel(res["mixed"]["vel"], res["fp64"]["vel"]) < 5e-2
 
 
+@pytest.mark.parametrize("dim,n", [(2, 20001), (3, 27001)])
+def test_mixed_random_cloud(O, W, dim, n):
+    # uniform random positions (no lattice, no hydrostatic cancellation), an
+    # odd particle count (the last pair record has an empty half), masses
+    # +-20%, densities rho0 (1 + N(0, 0.01^2)), velocities N(0, 0.5^2)
+    rng = np.random.default_rng(11)
+    dx = 0.02
+    side = dx * n ** (1.0 / dim)
+    p = W.dam_break_params(dim)
+    op = O.sim_params(dim, alpha=p.alpha, c_cfl=p.c_cfl, c0=p.c0, g=p.g, nu=p.nu)
+    s = O.State(dim, n)
+    s.x[:] = rng.uniform(0.0, side, (dim, n))
+    s.v[:] = rng.normal(0.0, 0.5, (dim, n))
+    s.dens[:] = p.rho0 * (1.0 + 0.01 * rng.normal(size=n))
+    s.mass[:] = p.rho0 * dx ** dim * (1.0 + 0.2 * rng.uniform(-1.0, 1.0, n))
+    s.h[:] = 1.3 * dx
+    s.eos(op)  # Tait pressure and sound speed from the densities, as the device's reorder pass does
+    s.force(op, 8)
+    out = {}
+    for prec in ("mixed", "fp32"):
+        with W.Simulation.from_state(p, state_dict(s), precision=prec) as sim:
+            sim.force()
+            out[prec] = (sim.download(), sim.stats())
+    m, st = out["mixed"]
+    e_acc, e_div = rel(m["acc"], s.a), rel(m["div_v"], s.divv)
+    print(f"mixed vs FP64 oracle, random cloud: acc {e_acc:.2e}, div_v {e_div:.2e}")
+    assert e_acc <= 5e-3 and e_div <= 5e-3
+    assert np.allclose(m["dt"], s.dt_i, rtol=5e-3)
+    n32 = out["fp32"][1]["pairs_last"]
+    assert abs(st["pairs_last"] - n32) <= 5e-3 * n32
+    f32 = out["fp32"][0]
+    assert rel(f32["acc"], s.a) <= 1e-5 and rel(f32["div_v"], s.divv) <= 1e-5
+
+
 def test_mixed_deterministic(W):
     p = W.dam_break_params(3)
     st = W.dam_break_state(W.dam_break_desc(3, 0.04, tank_l=2.0, col_h=1.0, tank_h=1.5), p)
