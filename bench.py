@@ -12,18 +12,23 @@ value = neighbour pair interactions evaluated in the timed steps / wall time
 of those steps (replayed from the library's two-step hipGraphs, as users
 step), for the mixed-precision path (packed FP16 pair geometry, FP32
 density, pressure and accumulation, FP64 integration); the FP32 plain path
-runs the same steps beside it ("paths", "mixed_over_fp32"). The roofline entry is for the
-headline force kernel, from HIP events recorded on the simulation's stream
-around each launch of a following window of up to 100 steps launched one by
-one. The CPU baseline times the FP64 oracle's force
-pass (the reference CPU functor, src/force.h:31-60, over cell-list
-candidates, OpenMP) on the same developed state, on the host's cores. The
+runs the same steps beside it ("paths", "mixed_over_fp32"). The default run
+is the configuration's 2,000 steps: 1,000 warm-up steps, then steps
+1,001-2,000 timed. The roofline entry is for the headline force kernel, from
+HIP events recorded on the simulation's stream around each launch of a
+following window of up to 100 steps launched one by one; the measured VALU
+FMA rates (FP32 and packed FP16) and the VALU issue fraction from the
+committed rocprofv3 summary sit beside it. The CPU baseline times the FP64
+oracle's force pass (the reference CPU functor, src/force.h:31-60, over
+cell-list candidates, OpenMP) on the same developed state, on the host's
+cores. The
 serving entry runs a seeded Poisson stream of 16 dam-break requests through
 the continuous-batching loop (p50/p99 completion latency).
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -62,6 +67,23 @@ def cpu_baseline(sim, params, budget_s, threads):
     return {"value": pairs / t, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
             "sample": f"{reps} FP64 force pass(es) of the oracle (src/force.h:31-60 over 27-cell candidates) "
                       f"on the developed {sim.n}-particle state, {t:.1f} s"}
+
+
+def cpu_baseline_c0(threads, steps=1000):
+    """The oracle's full 2D dam break at dx = 0.02 m (5,000 fluid + 1,518 wall
+    particles, alpha 0.1, CFL 0.25, FP64): init, then `steps` steps, each
+    with its dt, kick-drift, EoS, force pass and final kick."""
+    import oracle as O
+    s = O.State.dam_break(2, 0.02)
+    op = O.sim_params(2)
+    t0 = time.perf_counter()
+    s.init(op, threads)
+    pairs = sum(s.step(op, threads) for _ in range(steps))
+    t = time.perf_counter() - t0
+    return {"value": pairs / t, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
+            "steps_per_s": steps / t, "particles": s.n,
+            "sample": f"full run of the 2D dam break dx=0.02 ({s.n} particles, {steps} FP64 steps of the oracle: "
+                      f"src/main.cpp:89-133 step order, src/force.h:31-60 pair functor), {t:.2f} s"}
 
 
 def accuracy_study(W, dim, dx, marks, chunk, early=100):
@@ -314,8 +336,8 @@ def fdps_study(W, dim, dx, warmup, group_edge=5.0):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed", "fp64"],
                     help="headline path (the mixed and FP32 paths are both run unless --only)")
     ap.add_argument("--only", action="store_true", help="run the headline path alone")
@@ -419,11 +441,18 @@ def main():
         if prof:
             line["roofline"]["traffic"] = prof["bytes_per_launch"]
             line["roofline"]["traffic_source"] = prof["source"]
+            if "valu_issue_frac" in prof:
+                line["roofline"]["valu_issue_frac"] = prof["valu_issue_frac"]
     line["hbm_copy_GBps"] = W.copy_probe_gbps(1 << 30, 10)
+    # measured VALU FMA rates: the attainable peaks of the two instruction kinds the force passes use
+    peaks = {"fp32_fma": W.valu_probe_tflops("fp32"), "pk_f16_fma": W.valu_probe_tflops("pk_f16")}
+    line["valu_probe_TFLOPs"] = peaks
+    line["roofline"]["frac_of_measured"] = {k: achieved / v for k, v in peaks.items()}
     if rank == 0 and args.cpu_seconds > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"# cpu baseline on {threads} threads", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(keep, params, args.cpu_seconds, threads)
+        line["cpu_baseline_c0"] = cpu_baseline_c0(threads)
     keep.close()
     if rank == 0 and args.serve_requests > 0 and not args.only:
         print(f"# serving {args.serve_requests} requests", file=sys.stderr, flush=True)
@@ -499,7 +528,12 @@ def profiled_traffic(kernel):
     if len(vals) < 2:
         return None
     b = (2.0 * vals["FETCH_SIZE"] + vals["WRITE_SIZE"]) * 1024.0
-    return {"bytes_per_launch": b, "source": os.path.relpath(path, ROOT)}
+    out = {"bytes_per_launch": b, "source": os.path.relpath(path, ROOT)}
+    # "... so 84% of the 256 CUs x 1 wave64 VALU instruction per cycle issue rate"
+    m = re.search(r"so (\d+)% of the 256 CUs x 1 wave64 VALU instruction per cycle", open(path).read())
+    if m:
+        out["valu_issue_frac"] = int(m.group(1)) / 100.0
+    return out
 
 
 if __name__ == "__main__":

@@ -82,6 +82,7 @@ def lib():
         L.orc_ic_dam_break.argtypes = [C.c_int] + [C.c_double] * 6 + [C.c_int] + [C.c_double] * 5 + \
                                       [C.POINTER(_CState)]
         L.orc_sim_init.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), C.c_int]
+        L.orc_sim_step.restype = C.c_int64
         L.orc_sim_step.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), C.c_int]
         L.orc_sim_eos.argtypes = [C.POINTER(SimParams), C.POINTER(_CState)]
         L.orc_sim_force.restype = C.c_int64
@@ -251,7 +252,8 @@ class State:
         lib().orc_sim_init(C.byref(p), C.byref(self.c), nthreads)
 
     def step(self, p, nthreads=1):
-        lib().orc_sim_step(C.byref(p), C.byref(self.c), nthreads)
+        """One step; returns the pair interactions of its force pass."""
+        return lib().orc_sim_step(C.byref(p), C.byref(self.c), nthreads)
 
     def eos(self, p):
         lib().orc_sim_eos(C.byref(p), C.byref(self.c))

@@ -626,7 +626,7 @@ double orc_global_dt(const orc_sim_params* p, const orc_state* s) {
     return dt;
 }
 
-void orc_sim_step(const orc_sim_params* p, orc_state* s, int nthreads) {
+int64_t orc_sim_step(const orc_sim_params* p, orc_state* s, int nthreads) {
     /* src/main.cpp:89-133 */
     const int D = p->dim;
     const int64_t n = s->n;
@@ -645,9 +645,10 @@ void orc_sim_step(const orc_sim_params* p, orc_state* s, int nthreads) {
     s->time += dt;
     if (p->shepard_interval > 0 && s->step % p->shepard_interval == 0) sim_shepard(p, s, nthreads); /* :105 */
     orc_sim_eos(p, s);
-    orc_sim_force(p, s, nthreads);
+    const int64_t pairs = orc_sim_force(p, s, nthreads);
     sim_gravity_post(p, s, 1);
     s->step += 1;
+    return pairs;
 }
 
 void orc_conservation(const orc_sim_params* p, const orc_state* s, double* out) {

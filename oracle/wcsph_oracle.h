@@ -127,8 +127,9 @@ void orc_shepard(int dim, int64_t n, const double* x, const double* h, const dou
 /* CalcPressure, first force, the filter (if enabled) + gravity (src/main.cpp:69-79) */
 void orc_sim_init(const orc_sim_params* p, orc_state* s, int nthreads);
 /* one step of src/main.cpp:89-133; the filter on steps with
- * step % shepard_interval == 0, after the drift (src/main.cpp:105) */
-void orc_sim_step(const orc_sim_params* p, orc_state* s, int nthreads);
+ * step % shepard_interval == 0, after the drift (src/main.cpp:105);
+ * returns the true pair count of the step's force pass */
+int64_t orc_sim_step(const orc_sim_params* p, orc_state* s, int nthreads);
 /* getTimeStepGlobal (src/integral.h:2-12) */
 double orc_global_dt(const orc_sim_params* p, const orc_state* s);
 /* force pass over all particles with a cell list; returns true pair count */

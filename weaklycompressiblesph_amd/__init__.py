@@ -113,6 +113,15 @@ def copy_probe_gbps(nbytes=1 << 30, reps=10, device=0):
     return v.value
 
 
+def valu_probe_tflops(kind, reps=5, device=0):
+    """Measured VALU FMA rate in TFLOP/s: kind "fp32" (v_fma_f32) or
+    "pk_f16" (v_pk_fma_f16)."""
+    k = {"fp32": 0, "pk_f16": 1}[kind]
+    v = C.c_double()
+    check(lib().wcsph_valu_probe(device, k, reps, C.byref(v)))
+    return v.value
+
+
 def last_error():
     return lib().wcsph_last_error().decode()
 

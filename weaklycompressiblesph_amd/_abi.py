@@ -88,8 +88,8 @@ def res_dtype(dim):
 
 
 EXPORTS = [
-    "wcsph_params_reference", "wcsph_last_error", "wcsph_version", "wcsph_copy_probe", "wcsph_init",
-    "wcsph_finalize",
+    "wcsph_params_reference", "wcsph_last_error", "wcsph_version", "wcsph_copy_probe", "wcsph_valu_probe",
+    "wcsph_init", "wcsph_finalize",
     "wcsph_hydro_dispatch_3d", "wcsph_hydro_dispatch_2d", "wcsph_hydro_retrieve_3d", "wcsph_hydro_retrieve_2d",
     "wcsph_hydro_last_kernel_ms", "wcsph_hydro_last_timing", "wcsph_dam_break_count", "wcsph_dam_break_fill", "wcsph_sim_bytes",
     "wcsph_sim_create", "wcsph_sim_destroy", "wcsph_sim_upload", "wcsph_sim_init", "wcsph_sim_step",
@@ -119,6 +119,7 @@ def lib():
         "wcsph_last_error": (C.c_char_p, []),
         "wcsph_version": (C.c_char_p, []),
         "wcsph_copy_probe": (C.c_int, [I32, I64, I32, C.POINTER(F64)]),
+        "wcsph_valu_probe": (C.c_int, [I32, I32, I32, C.POINTER(F64)]),
         "wcsph_init": (C.c_int, [C.POINTER(Params)]),
         "wcsph_finalize": (C.c_int, []),
         "wcsph_hydro_dispatch_3d": (C.c_int, [I32, I32, VP, VP, VP, VP]),

@@ -1,11 +1,50 @@
-// HBM copy probe: a plain streaming copy of known size, used to measure the
-// attainable HBM bandwidth and to calibrate hardware byte counters.
+// Hardware probes.
+// - HBM copy probe: a plain streaming copy of known size, used to measure the
+//   attainable HBM bandwidth and to calibrate hardware byte counters.
+// - VALU probe: independent FMA chains in every lane of a full grid, timed
+//   with HIP events, giving the attainable rate of FP32 FMA (v_fma_f32; this
+//   build disables packed FP32) and of packed FP16 FMA (v_pk_fma_f16), the
+//   two instruction kinds the force passes are built from.
 #include "common.hpp"
 
 namespace {
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+constexpr int kValuChains = 8;  // independent chains per lane, enough to cover the FMA latency
+
+// KIND 0: x = x a + b in FP32; KIND 1: the same on half2 (two FMAs per op).
+// The chains converge to b / (1 - a), so no value overflows or goes subnormal.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_valu(float* __restrict__ out, int32_t iters, float a, float b) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float sum = 0.f;
+    if (KIND == 0) {
+        float x[kValuChains];
+#pragma unroll
+        for (int k = 0; k < kValuChains; ++k) x[k] = (float)(threadIdx.x + k) * 1e-3f;
+        for (int32_t it = 0; it < iters; ++it) {
+#pragma unroll
+            for (int k = 0; k < kValuChains; ++k) x[k] = __builtin_fmaf(x[k], a, b);
+        }
+#pragma unroll
+        for (int k = 0; k < kValuChains; ++k) sum += x[k];
+    } else {
+        const half2_t a2 = {(_Float16)a, (_Float16)a}, b2 = {(_Float16)b, (_Float16)b};
+        half2_t x[kValuChains];
+#pragma unroll
+        for (int k = 0; k < kValuChains; ++k) x[k] = half2_t{(_Float16)(threadIdx.x * 1e-3f), (_Float16)(k * 1e-3f)};
+        for (int32_t it = 0; it < iters; ++it) {
+#pragma unroll
+            for (int k = 0; k < kValuChains; ++k) x[k] = __builtin_elementwise_fma(x[k], a2, b2);
+        }
+#pragma unroll
+        for (int k = 0; k < kValuChains; ++k) sum += (float)x[k].x + (float)x[k].y;
+    }
+    out[gid] = sum;
 }
 }  // namespace
 
@@ -34,5 +73,44 @@ extern "C" int wcsph_copy_probe(int32_t device, int64_t bytes, int32_t reps, dou
         (void)hipEventDestroy(e1);
         (void)hipFree(a);
         (void)hipFree(b);
+    });
+}
+
+extern "C" int wcsph_valu_probe(int32_t device, int32_t kind, int32_t reps, double* tflops) {
+    return wcsph::guarded([&] {
+        WCSPH_REQUIRE((kind == 0 || kind == 1) && reps > 0 && tflops, WCSPH_EINVAL,
+                      "kind 0 (FP32 FMA) or 1 (packed FP16 FMA), reps > 0, tflops non-null");
+        WCSPH_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        WCSPH_HIP(hipGetDeviceProperties(&prop, device));
+        const int blocks = prop.multiProcessorCount * 8;  // 32 waves per CU
+        const int32_t iters = 1 << 15;
+        float* out = nullptr;
+        WCSPH_HIP(hipMalloc(&out, sizeof(float) * blocks * 256));
+        hipEvent_t e0, e1;
+        WCSPH_HIP(hipEventCreate(&e0));
+        WCSPH_HIP(hipEventCreate(&e1));
+        auto launch = [&] {
+            if (kind == 0)
+                hipLaunchKernelGGL(k_valu<0>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1.0e-3f);
+            else
+                hipLaunchKernelGGL(k_valu<1>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1.0e-3f);
+        };
+        launch();  // warm-up
+        float best = 1e30f;
+        for (int r = 0; r < reps; ++r) {
+            WCSPH_HIP(hipEventRecord(e0, 0));
+            launch();
+            WCSPH_HIP(hipEventRecord(e1, 0));
+            WCSPH_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            WCSPH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            best = ms < best ? ms : best;
+        }
+        const double fmas = (double)blocks * 256 * iters * kValuChains * (kind == 1 ? 2 : 1);
+        *tflops = 2.0 * fmas / (best * 1e-3) / 1e12;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(out);
     });
 }
