@@ -13,6 +13,18 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Device buffers and events of one probe call, released on every exit path.
+struct ProbeResources {
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~ProbeResources() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (void* p : buf)
+            if (p) (void)hipFree(p);
+    }
+};
+
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 constexpr int kValuChains = 8;  // independent chains per lane, enough to cover the FMA latency
 
@@ -53,26 +65,24 @@ extern "C" int wcsph_copy_probe(int32_t device, int64_t bytes, int32_t reps, dou
         WCSPH_REQUIRE(bytes >= 4096 && reps > 0 && gbps, WCSPH_EINVAL, "bytes >= 4096, reps > 0, gbps non-null");
         WCSPH_HIP(hipSetDevice(device));
         const int64_t n = bytes / 16;
-        float4 *a = nullptr, *b = nullptr;
-        WCSPH_HIP(hipMalloc(&a, n * 16));
-        WCSPH_HIP(hipMalloc(&b, n * 16));
-        hipEvent_t e0, e1;
-        WCSPH_HIP(hipEventCreate(&e0));
-        WCSPH_HIP(hipEventCreate(&e1));
+        ProbeResources r;
+        WCSPH_HIP(hipMalloc(&r.buf[0], n * 16));
+        WCSPH_HIP(hipMalloc(&r.buf[1], n * 16));
+        WCSPH_HIP(hipEventCreate(&r.ev[0]));
+        WCSPH_HIP(hipEventCreate(&r.ev[1]));
+        float4* a = static_cast<float4*>(r.buf[0]);
+        float4* b = static_cast<float4*>(r.buf[1]);
+        hipEvent_t e0 = r.ev[0], e1 = r.ev[1];
         WCSPH_HIP(hipMemset(a, 0, n * 16));
         const int blocks = 256 * 8 * 4;
         hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);  // warm-up
         WCSPH_HIP(hipEventRecord(e0, 0));
-        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+        for (int k = 0; k < reps; ++k) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
         WCSPH_HIP(hipEventRecord(e1, 0));
         WCSPH_HIP(hipEventSynchronize(e1));
         float ms = 0;
         WCSPH_HIP(hipEventElapsedTime(&ms, e0, e1));
         *gbps = 2.0 * n * 16 * reps / (ms * 1e-3) / 1e9;  // read + write
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipFree(a);
-        (void)hipFree(b);
     });
 }
 
@@ -85,11 +95,12 @@ extern "C" int wcsph_valu_probe(int32_t device, int32_t kind, int32_t reps, doub
         WCSPH_HIP(hipGetDeviceProperties(&prop, device));
         const int blocks = prop.multiProcessorCount * 8;  // 32 waves per CU
         const int32_t iters = 1 << 15;
-        float* out = nullptr;
-        WCSPH_HIP(hipMalloc(&out, sizeof(float) * blocks * 256));
-        hipEvent_t e0, e1;
-        WCSPH_HIP(hipEventCreate(&e0));
-        WCSPH_HIP(hipEventCreate(&e1));
+        ProbeResources r;
+        WCSPH_HIP(hipMalloc(&r.buf[0], sizeof(float) * blocks * 256));
+        WCSPH_HIP(hipEventCreate(&r.ev[0]));
+        WCSPH_HIP(hipEventCreate(&r.ev[1]));
+        float* out = static_cast<float*>(r.buf[0]);
+        hipEvent_t e0 = r.ev[0], e1 = r.ev[1];
         auto launch = [&] {
             if (kind == 0)
                 hipLaunchKernelGGL(k_valu<0>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1.0e-3f);
@@ -98,7 +109,7 @@ extern "C" int wcsph_valu_probe(int32_t device, int32_t kind, int32_t reps, doub
         };
         launch();  // warm-up
         float best = 1e30f;
-        for (int r = 0; r < reps; ++r) {
+        for (int k = 0; k < reps; ++k) {
             WCSPH_HIP(hipEventRecord(e0, 0));
             launch();
             WCSPH_HIP(hipEventRecord(e1, 0));
@@ -109,8 +120,5 @@ extern "C" int wcsph_valu_probe(int32_t device, int32_t kind, int32_t reps, doub
         }
         const double fmas = (double)blocks * 256 * iters * kValuChains * (kind == 1 ? 2 : 1);
         *tflops = 2.0 * fmas / (best * 1e-3) / 1e12;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipFree(out);
     });
 }
