@@ -200,3 +200,14 @@ def test_diagnostics_match_oracle(O, W):
     assert d["column"] == s.x[1][fl].max()
     fr = (s.type == 0) & (s.x[0] > 0.5)
     assert d["runup"] == s.x[1][fr].max()
+
+
+def test_valu_probe_rates(W):
+    # FP32 FMA: 256 CUs x 64 lanes x 2 flop per cycle, 78.6 TFLOP/s at
+    # 2.4 GHz (this build disables packed FP32); packed FP16 FMA does two per
+    # lane per instruction, so twice that at the same clock
+    f32 = W.valu_probe_tflops("fp32")
+    f16 = W.valu_probe_tflops("pk_f16")
+    print(f"VALU FMA: FP32 {f32:.1f} TFLOP/s, packed FP16 {f16:.1f} TFLOP/s")
+    assert 30.0 < f32 < 80.0
+    assert 1.7 < f16 / f32 < 2.3
