@@ -11,8 +11,9 @@
 //                                  (|x| <= L/2h, where FP16 has its finest
 //                                  steps) and the cell's x index
 //   VelRec16   {vx, vy, vz, m}     v / c0 and the scaled mass
-//   StateRec24 {rho[2], pr[2] (FP32), c/2 (half2)}  rho / rho0,
-//                                  (P / rho^2)(rho0 / c0^2), c / (2 c0)
+//   StateRec16 {rho[2], pr[2]}     FP32 rho / rho0 and (P / rho^2)(rho0 / c0^2)
+//   ch[m]                          half2 c / (2 c0), an array of its own
+// The three 16-byte records share one offset (m * 16) per candidate.
 // One 16-byte load therefore brings two candidates' positions, and the
 // geometric part of the pair body -- differences, r^2, dr.dv, the spline
 // factor -- is one packed FP16 VALU op (v_pk_fma_f16, v_pk_mul_f16 ...) for
@@ -47,17 +48,15 @@ struct alignas(16) PosRec16 {
 struct alignas(16) VelRec16 {
     h2 vx, vy, vz, m;
 };
-struct alignas(8) StateRec24 {
+struct alignas(16) StateRec16 {
     float rho[2], pr[2];
-    h2 ch;
-    uint32_t pad;
 };
-static_assert(sizeof(StateRec24) == 24, "state record layout");
 
 struct Mixed {
     PosRec16* pos;  // [(n + 1) / 2]
     VelRec16* vel;
-    StateRec24* st;
+    StateRec16* st;
+    h2* ch;
 };
 
 __host__ __device__ inline int64_t mixed_records(int64_t n) { return (n + 1) / 2; }
@@ -75,15 +74,15 @@ struct MixedConsts {
 __device__ __forceinline__ void write_mixed_state(const Mixed& mx, const MixedConsts& mc, int64_t p, double rho,
                                                   double pr, double c) {
     const int o = (int)(p & 1);
-    StateRec24& S = mx.st[p >> 1];
+    StateRec16& S = mx.st[p >> 1];
     S.rho[o] = (float)(rho * mc.inv_rho0);
     S.pr[o] = (float)(pr * mc.pr_scale);
-    reinterpret_cast<_Float16*>(&S.ch)[o] = (_Float16)(float)(0.5 * c * mc.inv_c0);
+    reinterpret_cast<_Float16*>(&mx.ch[p >> 1])[o] = (_Float16)(float)(0.5 * c * mc.inv_c0);
 }
 
 // The pair records written whole by k_reorder: lanes p and p ^ 1 (one wave)
-// swap their values and the even lane stores the three records. A lane past
-// the end supplies zeros. Every lane of the wave must call it.
+// swap their values and the even lane stores the records. A lane past the
+// end supplies zeros. Every lane of the wave must call it.
 template <int D>
 __device__ __forceinline__ void write_mixed_pair(const Mixed& mx, const MixedConsts& mc, int64_t p, bool live,
                                                  const float* xl, int cx, const float* vf, double rho, double pr,
@@ -116,14 +115,13 @@ __device__ __forceinline__ void write_mixed_pair(const Mixed& mx, const MixedCon
                                   hi(mine[1], other[1])};
         mx.vel[p >> 1] = VelRec16{lo(mine[2], other[2]), hi(mine[2], other[2]), lo(mine[3], other[3]),
                                   hi(mine[3], other[3])};
-        StateRec24 S;
+        StateRec16 S;
         S.rho[0] = __builtin_bit_cast(float, fs[0]);
         S.rho[1] = __builtin_bit_cast(float, ofs[0]);
         S.pr[0] = __builtin_bit_cast(float, fs[1]);
         S.pr[1] = __builtin_bit_cast(float, ofs[1]);
-        S.ch = as_h2(fs[2] | (ofs[2] << 16));
-        S.pad = 0u;
         mx.st[p >> 1] = S;
+        mx.ch[p >> 1] = as_h2(fs[2] | (ofs[2] << 16));
     }
 }
 
@@ -238,7 +236,7 @@ void k_nforce_mixed(NFMixedArgs a) {
     const bool odd = (ii & 1) != 0;
     const PosRec16 Pi = a.mx.pos[ii >> 1];
     const VelRec16 Vi = a.mx.vel[ii >> 1];
-    const StateRec24 Si = a.mx.st[ii >> 1];
+    const StateRec16 Si = a.mx.st[ii >> 1];
     auto pick = [&](h2 v) {
         const _Float16 s = odd ? v.y : v.x;
         return h2{s, s};
@@ -246,7 +244,7 @@ void k_nforce_mixed(NFMixedArgs a) {
     const h2 xi2 = pick(Pi.x), yi2 = pick(Pi.y), zi2 = pick(Pi.z), cxi2 = pick(Pi.cx);
     const h2 vxi = pick(Vi.vx), vyi = pick(Vi.vy), vzi = pick(Vi.vz);
     const float rhoi = odd ? Si.rho[1] : Si.rho[0], pri = odd ? Si.pr[1] : Si.pr[0];
-    const h2 chi2 = pick(Si.ch);
+    const h2 chi2 = pick(a.mx.ch[ii >> 1]);
     const float beta = a.beta;
     const h2 malpha2 = hsplat(a.malpha);
     const h2 Ls2 = a.k_Ls, eps2 = a.k_eps;
@@ -272,7 +270,8 @@ void k_nforce_mixed(NFMixedArgs a) {
         const uint4 T = etab[e >> kMixRecBits];
         const PosRec16 P = a.mx.pos[m];
         const VelRec16 V = a.mx.vel[m];
-        const StateRec24 St = a.mx.st[m];
+        const StateRec16 St = a.mx.st[m];
+        const h2 chj = a.mx.ch[m];
         total += T.w;
         const h2 dx = hfma(cxi2 - P.cx, Ls2, xi2 - P.x);
         const h2 dy = (yi2 - P.y) - as_h2(T.x);
@@ -295,7 +294,7 @@ void k_nforce_mixed(NFMixedArgs a) {
         const h2 inv_den = hrcp(r2 + eps2);     // 1 / (r^2 + eps h_i h_j)
         const h2 mu = hmin(drdv, zero2) * inv_den;  // (h_i + h_j)/2 = 1
         if (VS == 1) vsig2 = hmax(vsig2, -mu);
-        const h2 cbar = chi2 + St.ch;
+        const h2 cbar = chi2 + chj;
         const h2 amu = BT ? hfma(malpha2, cbar, hsplat(beta) * mu) * mu : malpha2 * cbar * mu;  // Pi (rho_i + rho_j)
         const h2 invr = hrsq(r2);
         const h2 rr = r2 * invr;               // r / h
@@ -306,15 +305,17 @@ void k_nforce_mixed(NFMixedArgs a) {
         const h2 mFv = mF * (r2 * inv_den);    // m_j F r^2 / den, the viscous factor before 1/(rho_i + rho_j)
         // density and pressure terms in FP32, one half at a time
         const float amus[2] = {(float)amu.x, (float)amu.y};
-        const float mFs[2] = {(float)mF.x, (float)mF.y};
-        const float mFvs[2] = {(float)mFv.x, (float)mFv.y};
+        const _Float16 mFh[2] = {mF.x, mF.y};
+        const _Float16 mFvh[2] = {mFv.x, mFv.y};
         const float dxs[2][3] = {{(float)dx.x, (float)dy.x, (float)dz.x}, {(float)dx.y, (float)dy.y, (float)dz.y}};
         const float dvs[2][3] = {{(float)dvx.x, (float)dvy.x, (float)dvz.x}, {(float)dvx.y, (float)dvy.y, (float)dvz.y}};
 #pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
             const float inv_rs = __builtin_amdgcn_rcpf(rhoi + St.rho[hh]);  // 1 / (rho_i + rho_j)
-            const float cp = -(mFs[hh] * fmaf(amus[hh], inv_rs, pri + St.pr[hh]));
-            const float vc = mFvs[hh] * inv_rs;
+            // FP16 x FP32 products written as fma(., ., 0): one v_fma_mix_f32
+            // each, equal to the product up to the sign of an exact zero
+            const float cp = fmaf((float)mFh[hh], -fmaf(amus[hh], inv_rs, pri + St.pr[hh]), 0.0f);
+            const float vc = fmaf((float)mFvh[hh], inv_rs, 0.0f);
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 accp[d] = fmaf(cp, dxs[hh][d], accp[d]);

@@ -608,11 +608,13 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         if (s->cfg.precision == WCSPH_PREC_MIXED) {
             s->mx.pos = s->alloc<PosRec16>(mixed_records(n));
             s->mx.vel = s->alloc<VelRec16>(mixed_records(n));
-            s->mx.st = s->alloc<StateRec24>(mixed_records(n));
+            s->mx.st = s->alloc<StateRec16>(mixed_records(n));
+            s->mx.ch = s->alloc<h2>(mixed_records(n));
             // the high half of the last record stays zero when n is odd
             WCSPH_HIP(hipMemsetAsync(s->mx.pos, 0, sizeof(PosRec16) * mixed_records(n), s->st));
             WCSPH_HIP(hipMemsetAsync(s->mx.vel, 0, sizeof(VelRec16) * mixed_records(n), s->st));
-            WCSPH_HIP(hipMemsetAsync(s->mx.st, 0, sizeof(StateRec24) * mixed_records(n), s->st));
+            WCSPH_HIP(hipMemsetAsync(s->mx.st, 0, sizeof(StateRec16) * mixed_records(n), s->st));
+            WCSPH_HIP(hipMemsetAsync(s->mx.ch, 0, sizeof(h2) * mixed_records(n), s->st));
         }
         s->key = s->alloc<uint32_t>(n);
         s->slot = s->alloc<uint32_t>(n);
