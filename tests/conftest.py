@@ -24,3 +24,29 @@ def W():
     import weaklycompressiblesph_amd as W
     W._abi.lib()
     return W
+
+
+@pytest.fixture(scope="session")
+def dt_neighbours(O):
+    """Per-particle dt of src/force.h:56 with v_sig taken over the true
+    neighbours (the device predicate, via the oracle's cell-local brute
+    force): dt_neighbours(s, op, (lo, L, nc)). The device-resident passes
+    use this set; the oracle's force pass takes v_sig over all 27-cell
+    candidates, as the reference does over the lists it is handed, which
+    differs in v_sig mode 1 (max of -mu)."""
+    import numpy as np
+
+    def dt(s, op, grid):
+        lo, L, nc = grid
+        counts, lists = O.neighbours_brute_cells(s.x, s.h.astype(np.float32), lo, L, nc, cap=160)
+        assert counts.max() < 160
+        I = np.repeat(np.arange(s.n), counts)
+        J = lists[np.arange(lists.shape[1])[None, :] < counts[:, None]]
+        dr, dv = s.x[:, I] - s.x[:, J], s.v[:, I] - s.v[:, J]
+        r2, drdv = (dr * dr).sum(0), (dr * dv).sum(0)
+        hi, hj = s.h[I], s.h[J]
+        mu = np.where(drdv < 0.0, 0.5 * (hi + hj) * drdv / (r2 + op.eps * hi * hj), 0.0)
+        vsig = np.zeros(s.n)
+        np.maximum.at(vsig, I, -mu if op.vsig_mode else mu)
+        return op.c_cfl * s.h / (s.snds + vsig)
+    return dt

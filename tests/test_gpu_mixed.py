@@ -38,33 +38,12 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def dt_over_neighbours(O, s, op, grid):
-    """Per-particle dt of src/force.h:56 with v_sig taken over the true
-    neighbours (the device predicate, via the oracle's cell-local brute
-    force). The device-resident passes use this set; the oracle's force pass
-    takes v_sig over all 27-cell candidates, as the reference does over the
-    lists it is given, which differs in v_sig mode 1 (max of -mu)."""
-    lo, L, nc = grid
-    counts, lists = O.neighbours_brute_cells(s.x, s.h.astype(np.float32), lo, L, nc, cap=160)
-    assert counts.max() < 160
-    I = np.repeat(np.arange(s.n), counts)
-    J = lists[np.arange(lists.shape[1])[None, :] < counts[:, None]]
-    dr, dv = s.x[:, I] - s.x[:, J], s.v[:, I] - s.v[:, J]
-    r2, drdv = (dr * dr).sum(0), (dr * dv).sum(0)
-    hi, hj = s.h[I], s.h[J]
-    mu = np.where(drdv < 0.0, 0.5 * (hi + hj) * drdv / (r2 + op.eps * hi * hj), 0.0)
-    vs = -mu if op.vsig_mode else mu
-    vsig = np.zeros(s.n)
-    np.maximum.at(vsig, I, vs)
-    return op.c_cfl * s.h / (s.snds + vsig)
-
-
 SMALL3 = dict(tank_l=2.0, col_h=1.0, tank_h=1.5)
 
 
 @pytest.mark.parametrize("vsig_mode", [0, 1])
 @pytest.mark.parametrize("dim,dx,geom", [(2, 0.01, {}), (3, 0.03, SMALL3)])
-def test_mixed_force_pass_error(O, W, dim, dx, geom, vsig_mode):
+def test_mixed_force_pass_error(O, W, dt_neighbours, dim, dx, geom, vsig_mode):
     s, p, op = developed(O, W, dim, dx, 150, **geom)
     op.vsig_mode = vsig_mode
     p.vsig_mode = vsig_mode
@@ -79,7 +58,7 @@ def test_mixed_force_pass_error(O, W, dim, dx, geom, vsig_mode):
     e_acc, e_div = rel(m["acc"], s.a), rel(m["div_v"], s.divv)
     print(f"mixed vs FP64 oracle: acc {e_acc:.2e}, div_v {e_div:.2e}")
     assert e_acc <= 1e-2 and e_div <= 5e-3
-    assert np.allclose(m["dt"], dt_over_neighbours(O, s, op, grid), rtol=5e-3)
+    assert np.allclose(m["dt"], dt_neighbours(s, op, grid), rtol=5e-3)
     n32 = out["fp32"][1]["pairs_last"]
     assert abs(st["pairs_last"] - n32) <= 5e-3 * n32
 
@@ -149,7 +128,7 @@ def developed_3d(O, W):
 @pytest.mark.parametrize("precision,tol_acc,tol_div", [("mixed", 1e-2, 5e-3), ("fp32", 1e-5, 1e-5)])
 @pytest.mark.parametrize("cps", [1, 2])
 @pytest.mark.parametrize("beta,vsig", [(0.0, 1), (0.2, 0), (0.2, 1)])
-def test_force_variants_beta_vsig_cells(O, W, precision, tol_acc, tol_div, cps, beta, vsig):
+def test_force_variants_beta_vsig_cells(O, W, dt_neighbours, precision, tol_acc, tol_div, cps, beta, vsig):
     # the (cells per support, beta != 0, v_sig mode) instances of the force
     # passes on one developed 3D state, each against the oracle with the
     # same parameters
@@ -166,7 +145,7 @@ def test_force_variants_beta_vsig_cells(O, W, precision, tol_acc, tol_div, cps, 
         m = sim.download()
         grid = sim.grid()
     assert rel(m["acc"], s.a) <= tol_acc and rel(m["div_v"], s.divv) <= tol_div
-    assert np.allclose(m["dt"], dt_over_neighbours(O, s, op, grid), rtol=5e-3 if precision == "mixed" else 1e-6)
+    assert np.allclose(m["dt"], dt_neighbours(s, op, grid), rtol=5e-3 if precision == "mixed" else 1e-6)
 
 
 def test_mixed_deterministic(W):

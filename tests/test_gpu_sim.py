@@ -97,17 +97,23 @@ def test_large_sparse_grid(O, W, precision, tol):
     assert rel(out["div_v"], s.divv) <= tol
 
 
-def test_force_pass_with_variable_h_and_m(O, W):
+@pytest.mark.parametrize("vsig_mode", [0, 1])
+def test_force_pass_with_variable_h_and_m(O, W, dt_neighbours, vsig_mode):
+    # the general pair body (per-pair mean h), in both v_sig modes
     s, p, op = developed(O, W, 3, 0.05, 10, **SMALL3)
     rng = np.random.default_rng(5)
     s.h *= 1 + 0.15 * rng.uniform(-1, 1, s.n)
     s.mass *= 1 + 0.3 * rng.uniform(-1, 1, s.n)
+    op.vsig_mode = vsig_mode
+    p.vsig_mode = vsig_mode
     s.force(op, 8)
     with W.Simulation.from_state(p, state_dict(s)) as sim:
         sim.force()
         out = sim.download()
+        grid = sim.grid()
     assert rel(out["acc"], s.a) <= 1e-5
     assert rel(out["div_v"], s.divv) <= 1e-5
+    assert np.allclose(out["dt"], dt_neighbours(s, op, grid), rtol=1e-6)
 
 
 def test_steps_match_oracle_fp64(O, W):
