@@ -115,39 +115,6 @@ def test_mixed_random_cloud(O, W, dim, n):
     assert rel(f32["acc"], s.a) <= 1e-5 and rel(f32["div_v"], s.divv) <= 1e-5
 
 
-_DEVELOPED = {}
-
-
-def developed_3d(O, W):
-    # one developed 3D state shared by the variant cases below
-    if "s" not in _DEVELOPED:
-        _DEVELOPED["s"] = developed(O, W, 3, 0.03, 100, **SMALL3)
-    return _DEVELOPED["s"]
-
-
-@pytest.mark.parametrize("precision,tol_acc,tol_div", [("mixed", 1e-2, 5e-3), ("fp32", 1e-5, 1e-5)])
-@pytest.mark.parametrize("cps", [1, 2])
-@pytest.mark.parametrize("beta,vsig", [(0.0, 1), (0.2, 0), (0.2, 1)])
-def test_force_variants_beta_vsig_cells(O, W, dt_neighbours, precision, tol_acc, tol_div, cps, beta, vsig):
-    # the (cells per support, beta != 0, v_sig mode) instances of the force
-    # passes on one developed 3D state, each against the oracle with the
-    # same parameters
-    s0, p0, _ = developed_3d(O, W)
-    p = W.dam_break_params(3, beta=beta, vsig_mode=vsig)
-    op = O.sim_params(3, alpha=p.alpha, beta=beta, vsig_mode=vsig, c_cfl=p.c_cfl, c0=p.c0, g=p.g, nu=p.nu)
-    s = O.State(3, s0.n)
-    s.x[:], s.v[:], s.dens[:], s.mass[:], s.h[:], s.type[:] = s0.x, s0.v, s0.dens, s0.mass, s0.h, s0.type
-    s.eos(op)
-    s.force(op, 8)
-    print(f"case {precision} cps={cps} beta={beta} vsig={vsig}", flush=True)
-    with W.Simulation.from_state(p, state_dict(s), precision=precision, cells_per_support=cps) as sim:
-        sim.force()
-        m = sim.download()
-        grid = sim.grid()
-    assert rel(m["acc"], s.a) <= tol_acc and rel(m["div_v"], s.divv) <= tol_div
-    assert np.allclose(m["dt"], dt_neighbours(s, op, grid), rtol=5e-3 if precision == "mixed" else 1e-6)
-
-
 def test_mixed_deterministic(W):
     p = W.dam_break_params(3)
     st = W.dam_break_state(W.dam_break_desc(3, 0.04, tank_l=2.0, col_h=1.0, tank_h=1.5), p)
