@@ -186,6 +186,41 @@ def test_density_floor_matches_oracle(O, W, precision, tol):
     assert rel(out["dens"] - st0["dens"], s.dens - st0["dens"]) < tol
 
 
+def test_full_size_momentum_and_pairs(W):
+    # the 3D dam break at dx = 0.0126 (1,065,378 particles), developed for
+    # 200 steps: the symmetric pair forces cancel in the total hydro momentum
+    # (sum m_i a_i = 0 for any masses: every pressure, viscosity and
+    # artificial-viscosity term is antisymmetric in i, j), the FP32 and FP64
+    # paths count every ordered pair from both ends, and the mixed path's
+    # FP16 predicate finds the FP32 path's pairs to within 0.5% (it tests
+    # (y_i - row offset) - y_j, which rounds differently from j's side, so a
+    # pair within rounding of the support edge may be counted from one end)
+    p = W.dam_break_params(3)
+    st = W.dam_break_state(W.dam_break_desc(3, 0.0126), p)
+    n = len(st["dens"])
+    assert n == 1065378
+    with W.Simulation.from_state(p, st, precision="fp32") as sim:
+        sim.init()
+        sim.step(200)
+        d = sim.download()
+    state = dict(pos=d["pos"], vel=d["vel"], dens=d["dens"], mass=st["mass"], h=st["h"], type=st["type"])
+    res = {}
+    for prec, tol in (("fp64", 1e-12), ("fp32", 1e-6), ("mixed", 2e-5)):
+        with W.Simulation.from_state(p, state, precision=prec) as sim:
+            sim.force()
+            out, stats = sim.download(), sim.stats()
+        ma = out["acc"] * st["mass"]
+        imbalance = np.linalg.norm(ma.sum(axis=1)) / np.abs(ma).sum()
+        print(f"{prec}: |sum m a| / sum |m a| = {imbalance:.2e}, pairs {stats['pairs_last']}", flush=True)
+        assert imbalance <= tol, prec
+        assert stats["escaped"] == 0
+        if prec != "mixed":
+            assert stats["pairs_last"] % 2 == 0
+        res[prec] = stats["pairs_last"]
+    assert res["fp64"] == res["fp32"]
+    assert abs(res["mixed"] - res["fp32"]) <= 5e-3 * res["fp32"]
+
+
 @pytest.mark.parametrize("precision,dt_tol", [("fp32", 1e-6), ("fp64", 1e-12)])
 def test_free_fall_closed_form(W, precision, dt_tol):
     # one isolated particle: dt = min(C h / c, 0.25 sqrt(h/g)) (the FP32 path
