@@ -221,6 +221,33 @@ def test_full_size_momentum_and_pairs(W):
     assert abs(res["mixed"] - res["fp32"]) <= 5e-3 * res["fp32"]
 
 
+def test_16m_momentum_and_pairs(W):
+    # the same properties at 16.0M particles (dx = 0.00443, 100 steps): FP32
+    # and mixed paths
+    p = W.dam_break_params(3)
+    st = W.dam_break_state(W.dam_break_desc(3, 0.00443), p)
+    assert len(st["dens"]) == 15993515
+    with W.Simulation.from_state(p, st, precision="fp32") as sim:
+        sim.init()
+        sim.step(100)
+        d = sim.download()
+    st.update(pos=d["pos"], vel=d["vel"], dens=d["dens"])
+    del d
+    res = {}
+    for prec, tol in (("fp32", 1e-6), ("mixed", 2e-5)):
+        with W.Simulation.from_state(p, st, precision=prec) as sim:
+            sim.force()
+            out, stats = sim.download(), sim.stats()
+        ma = out["acc"] * st["mass"]
+        imbalance = np.linalg.norm(ma.sum(axis=1)) / np.abs(ma).sum()
+        print(f"{prec}: |sum m a| / sum |m a| = {imbalance:.2e}, pairs {stats['pairs_last']}", flush=True)
+        assert imbalance <= tol, prec
+        assert stats["escaped"] == 0
+        res[prec] = stats["pairs_last"]
+    assert res["fp32"] % 2 == 0
+    assert abs(res["mixed"] - res["fp32"]) <= 5e-3 * res["fp32"]
+
+
 @pytest.mark.parametrize("precision,dt_tol", [("fp32", 1e-6), ("fp64", 1e-12)])
 def test_free_fall_closed_form(W, precision, dt_tol):
     # one isolated particle: dt = min(C h / c, 0.25 sqrt(h/g)) (the FP32 path
