@@ -90,35 +90,50 @@ def accuracy_study(W, dim, dx, marks, chunk, early=100):
     """Mixed and FP32 paths against the FP64 GPU path from the same initial
     condition: dam front (max x of fluid within 2 dx of the floor), residual
     column height (fluid within 2 dx of the left wall), run-up on the far
-    wall (fluid within 2 dx of it) at the time marks, energy drift, and
-    particle-wise errors after `early` steps (later, particle trajectories of
-    any two precisions decorrelate). Returns one JSON-able dict."""
+    wall (fluid within 2 dx of it) at the time marks, the same three over the
+    bulk fluid (rho >= 0.9 rho0, without spray) at the first step past each
+    mark, energy drift, and particle-wise errors after `early` steps (later,
+    particle trajectories of any two precisions decorrelate). Returns one
+    JSON-able dict."""
     import numpy as np
     p = W.dam_break_params(dim)
     desc = W.dam_break_desc(dim, dx)
     st = W.dam_break_state(desc, p)
     band, xwall = 2 * dx, desc.tank_l - 2 * dx
+    fluid = st["type"] == 0
     runs = {}
+
+    def bulk(d):
+        # the same three measures over the bulk fluid (rho >= 0.9 rho0): spray
+        # particles, which have expanded to a fraction of rho0, are left out
+        x, z = d["pos"][0], d["pos"][dim - 1]
+        b = fluid & (d["dens"] >= 0.9 * p.rho0)
+        sel = lambda m, v: float(v[m].max()) if m.any() else -1e300
+        return [sel(b & (z < band), x), sel(b & (x < band), z), sel(b & (x > xwall), z)]
+
     for prec in ("fp64", "fp32", "mixed"):
         t_start = time.perf_counter()
         sim = W.Simulation.from_state(p, st, precision=prec)
         sim.init()
-        rows, todo, last = [], list(marks), time.time()
+        rows, todo, last, bulk_rows = [], list(marks), time.time(), []
         sim.step(early)  # particle-wise errors are compared here, before the flow
         snap = sim.download()  # becomes sensitive to rounding
         while todo:
-            sim.step(chunk)
+            s = sim.stats()
+            # single steps into each mark, so every path samples it within one dt
+            sim.step(1 if s["time"] + (chunk + 1) * s["dt"] >= todo[0] else chunk)
             s = sim.stats()
             d = sim.diagnostics(band, band, xwall)
             rows.append([s["time"], d["front"], d["column"], d["runup"], d["kinetic"] + d["potential"] + d["internal"],
                          d["mass"], s["step"]])
             if s["time"] >= todo[0]:
+                bulk_rows.append([s["time"]] + bulk(sim.download()))
                 todo.pop(0)
             if time.time() - last > 15:
                 print(f"# {prec}: t={s['time']:.3f} s, step {s['step']}", file=sys.stderr, flush=True)
                 last = time.time()
         runs[prec] = {"rows": np.array(rows), "snap": snap, "wall_s": time.perf_counter() - t_start,
-                      "steps": int(rows[-1][6])}
+                      "steps": int(rows[-1][6]), "bulk": np.array(bulk_rows)}
         sim.close()
     ref = runs["fp64"]
     out = {"workload": f"{dim}D dam break dx={dx}, {len(st['dens'])} particles", "marks_s": list(marks), "paths": {}}
@@ -133,6 +148,15 @@ def accuracy_study(W, dim, dx, marks, chunk, early=100):
             if prec != "fp64":
                 rv = np.interp(marks, ref["rows"][:, 0], ref["rows"][:, k])
                 entry[name + "_rel_err"] = [float(abs(a - b) / abs(b)) for a, b in zip(vals, rv)]
+        Bk = r["bulk"]
+        entry["bulk_at_marks"] = {"time": [float(t) for t in Bk[:, 0]],
+                                  **{name: [float(v) for v in Bk[:, k]] for k, name in
+                                     ((1, "front"), (2, "column"), (3, "runup"))}}
+        if prec != "fp64":
+            Rb = ref["bulk"]
+            for k, name in ((1, "front"), (2, "column"), (3, "runup")):
+                entry["bulk_at_marks"][name + "_rel_err"] = [
+                    float(abs(a - b) / abs(b)) if b > -1e299 and a > -1e299 else None for a, b in zip(Bk[:, k], Rb[:, k])]
         if prec != "fp64":
             a, b = r["snap"], ref["snap"]
             disp = b["pos"] - st["pos"]
