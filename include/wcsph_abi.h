@@ -72,12 +72,15 @@ typedef struct {
 /* Fill p with the reference CPU functor's constants (src/force.h:32,45-47,
  * src/param.h:5, src/init/Dam2.h:38-43,106, src/EoS.h:38). */
 int wcsph_params_reference(int32_t dim, wcsph_params* p);
+/* Message of the calling thread's last failure. No counterpart in the
+ * reference, which asserts instead (src/kernel.cu:316,367). */
 const char* wcsph_last_error(void);
-/* Library version string, for logs. */
+/* Library version string, for logs. No counterpart in the reference. */
 const char* wcsph_version(void);
 /* HBM bandwidth probe: `reps` streaming copies of `bytes` on `device`;
  * *gbps = (read + write bytes) / time. Used by the bench for the measured
- * HBM peak and to calibrate hardware byte counters (kernel k_copy). */
+ * HBM peak and to calibrate hardware byte counters (kernel k_copy). No
+ * counterpart in the reference. */
 int wcsph_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps);
 /* VALU rate probe: independent FMA chains in every lane of a full grid on
  * `device`, best of `reps` launches; *tflops = 2 x FMAs / time. kind 0: FP32
@@ -89,6 +92,8 @@ int wcsph_valu_probe(int32_t device, int32_t kind, int32_t reps, double* tflops)
 /* ---------------- 1. FDPS multiwalk drop-in ---------------- */
 /* Replaces the lazy first-call set-up of src/kernel.cu:290-305. */
 int wcsph_init(const wcsph_params* p);
+/* Releases the drop-in's buffers and streams, which the reference keeps as
+ * static singletons for the life of the process (src/kernel.cu:42-131,239). */
 int wcsph_finalize(void);
 
 /* GPU_WC::HydrDispatchKernel (src/kernel.cu:289-381). Copies the walks into
@@ -98,21 +103,24 @@ int wcsph_finalize(void);
 int wcsph_hydro_dispatch_3d(int32_t tag, int32_t n_walk, const wcsph_epi_hydro_3d* const* epi,
                             const int32_t* n_epi, const wcsph_epj_hydro_3d* const* epj,
                             const int32_t* n_epj);
+/* The same for a 2D FDPS build (F64vec of two doubles, src/class.h:167-176,
+ * 212-223 with PARTICLE_SIMULATOR_TWO_DIMENSION). */
 int wcsph_hydro_dispatch_2d(int32_t tag, int32_t n_walk, const wcsph_epi_hydro_2d* const* epi,
                             const int32_t* n_epi, const wcsph_epj_hydro_2d* const* epj,
                             const int32_t* n_epj);
 /* GPU_WC::HydrRetrieveKernel (src/kernel.cu:383-409). Waits for the tag's
  * batch and assigns acc, dt and div_v of every i of every walk. */
 int wcsph_hydro_retrieve_3d(int32_t tag, int32_t n_walk, const int32_t* ni, wcsph_result_hydro_3d** force);
+/* The same for a 2D FDPS build (RESULT::Hydro of 32 B, src/class.h:63-67). */
 int wcsph_hydro_retrieve_2d(int32_t tag, int32_t n_walk, const int32_t* ni, wcsph_result_hydro_2d** force);
 /* Device time (ms) of the last batch's force kernels of a tag, from HIP
- * events. */
+ * events. No counterpart in the reference (its CudaTimer is unused). */
 int wcsph_hydro_last_kernel_ms(int32_t tag, float* ms);
 /* Timing of the last batch of a tag: out[0] host packing (ms, wall clock,
  * all chunks), out[1] device span from the first upload to the end of the
  * result copy (ms, events), out[2] force kernels (ms, events), out[3] host
  * unpacking in retrieve (ms), out[4] chunks the batch was cut into. Waits
- * for the batch's device work. */
+ * for the batch's device work. No counterpart in the reference. */
 int wcsph_hydro_last_timing(int32_t tag, double* out);
 
 /* ---------------- 2. device-resident simulation ---------------- */
@@ -153,14 +161,20 @@ typedef struct {
     double dx, col_w, col_h, depth, tank_l, tank_h, hfac;
 } wcsph_dam_break_desc;
 
-/* Number of particles the description generates (fluid and wall). */
+/* Number of particles the description generates (fluid and wall). Replaces
+ * the lattice loops of src/init/Dam2.h:9-103 for the configurations' tank. */
 int wcsph_dam_break_count(const wcsph_dam_break_desc* d, int64_t* n_fluid, int64_t* n_wall);
-/* Host-side generation into caller SoA arrays: pos/vel [dim][n], others [n]. */
+/* Host-side generation into caller SoA arrays: pos/vel [dim][n], others [n]
+ * (src/init/Dam2.h:9-103; the density is hydrostatic, which the reference
+ * leaves commented out at src/init/Dam2.h:73-74). */
 int wcsph_dam_break_fill(const wcsph_dam_break_desc* d, const wcsph_params* p, double* pos,
                          double* vel, double* dens, double* mass, double* h, int32_t* type);
-/* Device bytes a simulation of n particles needs (for admission control). */
+/* Device bytes a simulation of n particles needs (for admission control).
+ * No counterpart in the reference. */
 int64_t wcsph_sim_bytes(int32_t dim, int64_t n, const wcsph_sim_config* cfg);
 
+/* Create / destroy the device state of one simulation: the role of FDPS's
+ * ParticleSystem and tree set-up in src/main.cpp:33-67, held in HBM. */
 int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t n, wcsph_sim** out);
 int wcsph_sim_destroy(wcsph_sim* s);
 /* Upload a state (SoA, host): pos/vel [dim][n], dens/mass/h [n], type [n]
@@ -177,22 +191,30 @@ int wcsph_sim_init(wcsph_sim* s);
  * (src/main.cpp:105, src/param.h:12). */
 int wcsph_sim_step(wcsph_sim* s, int32_t nsteps);
 /* Steps are replayed from a captured two-step hipGraph (one submission per
- * two steps) unless per-step timing is on or graphs are disabled here. */
+ * two steps) unless per-step timing is on or graphs are disabled here. No
+ * counterpart in the reference. */
 int wcsph_sim_set_graphs(wcsph_sim* s, int32_t enable);
-/* Non-blocking: *idle = 1 when all queued work of the simulation is done. */
+/* Non-blocking: *idle = 1 when all queued work of the simulation is done.
+ * No counterpart in the reference. */
 int wcsph_sim_query(wcsph_sim* s, int32_t* idle);
 /* Re-bin and run one force pass on the current state (no integration):
- * acc (hydro only, no gravity), div_v and dt_i of the current positions. */
+ * acc (hydro only, no gravity), div_v and dt_i of the current positions --
+ * CalcPressure and calcForceAllAndWriteBack of src/main.cpp:70-75. */
 int wcsph_sim_force(wcsph_sim* s);
+/* Wait for the simulation's queued work. No counterpart in the reference. */
 int wcsph_sim_sync(wcsph_sim* s);
+/* Clock, pair counts and bounds of the last pass; the time and step are the
+ * reference's sysinfo (src/main.cpp:89-96). */
 int wcsph_sim_get_stats(wcsph_sim* s, wcsph_sim_stats* st);
 /* Download in the original particle order (index given at upload). Any
  * pointer may be NULL. acc is hydro + gravity after a step, hydro only after
- * wcsph_sim_force. */
+ * wcsph_sim_force. Plays the part of reading FDPS's ParticleSystem. */
 int wcsph_sim_download(wcsph_sim* s, double* pos, double* vel, double* dens, double* acc,
                        double* div_v, double* dt_i);
 /* Neighbour counts (and lists of original indices, ascending, if lists is
- * not NULL) of the last force pass, in original order; lists is [n][cap]. */
+ * not NULL) of the last force pass, in original order; lists is [n][cap].
+ * The set the functor's support implies (src/force.h:49); FDPS exposes only
+ * superset lists. */
 int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t cap);
 /* FP64 reductions on the device: out[0] mass, out[1..3] momentum,
  * out[4] kinetic, out[5] potential, out[6] internal energy of the clamped
@@ -200,19 +222,23 @@ int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t 
  * out[7] dam front (max x of fluid with vertical coordinate < zfront),
  * out[8] column height (max vertical coordinate of fluid with x < xcol),
  * out[9] run-up (max vertical coordinate of fluid with x > xwall). */
+/* No counterpart in the reference, which records neither conservation sums
+ * nor dam-front measures. */
 int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double xwall, double* out);
 /* The cell grid (for checking the neighbour predicate, which is defined
- * on cell-local coordinates): lo[3], cell edge *L, cells per axis nc[3]. */
+ * on cell-local coordinates): lo[3], cell edge *L, cells per axis nc[3].
+ * No counterpart in the reference (FDPS builds a tree instead). */
 int wcsph_sim_grid(wcsph_sim* s, double* lo, double* L, int32_t* nc);
 /* Device time (ms) of the kernels of the last wcsph_sim_force call:
  * out[0] binning + reorder, out[1] separate neighbour pass (FP64 path;
- * 0 when fused), out[2] force pass. */
+ * 0 when fused), out[2] force pass. No counterpart in the reference. */
 int wcsph_sim_last_force_ms(wcsph_sim* s, float* out);
 /* Per-step HIP-event timing on the simulation's stream. While enabled, each
  * step records events around its stages; wcsph_sim_timing waits, returns the
  * summed device times (ms) since enabling -- out[0] kick-drift + binning +
  * reorder, out[1] neighbour pass, out[2] force pass, out[3] final stage,
- * out[4] whole steps -- with *nsteps timed, and restarts the sums. */
+ * out[4] whole steps -- with *nsteps timed, and restarts the sums. The
+ * reference only prints the wall time of each step (src/main.cpp:94-96). */
 int wcsph_sim_set_timing(wcsph_sim* s, int32_t enable);
 int wcsph_sim_timing(wcsph_sim* s, double* out, int64_t* nsteps);
 
@@ -248,7 +274,8 @@ typedef struct {
  * is free and the budget allows, step each admitted request on its own
  * stream from a captured two-step graph, retire finished ones and record
  * per-request timing (latency = end_s - arrival_s) in out[] (same order as
- * reqs). Returns when every request is done. */
+ * reqs). Returns when every request is done. No counterpart in the
+ * reference, which runs one simulation per process. */
 int wcsph_serve(const wcsph_serve_config* cfg, const wcsph_request* reqs, int32_t n, wcsph_request_result* out);
 
 /* ---------------- 4. snapshots and restart ---------------- */
@@ -284,14 +311,16 @@ int wcsph_sim_write_ascii(wcsph_sim* s, const char* path);
  * pointers and so cannot be read by another process. */
 int wcsph_sim_save(wcsph_sim* s, const char* path);
 /* Create a simulation on `device` from a restart file. Stepping it continues
- * bit for bit as the saved simulation would have continued. */
+ * bit for bit as the saved simulation would have continued. Replaces
+ * InputBinary (src/io.h:60-84). */
 int wcsph_sim_restore(const char* path, int32_t device, wcsph_sim** out);
 /* Set the clock after wcsph_sim_init (for a state read back from a
  * snapshot: the reference resumes at the file's time and step + 1,
  * src/io.h:60-84, and the Shepard schedule follows the step index). */
 int wcsph_sim_set_clock(wcsph_sim* s, double time, int64_t step);
 /* The parameters, configuration and particle count a simulation was created
- * with (for one made by wcsph_sim_restore); any pointer may be NULL. */
+ * with (for one made by wcsph_sim_restore); any pointer may be NULL. No
+ * counterpart in the reference. */
 int wcsph_sim_params(wcsph_sim* s, wcsph_params* p, wcsph_sim_config* cfg, int64_t* n);
 
 #ifdef __cplusplus
