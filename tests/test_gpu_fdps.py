@@ -148,6 +148,16 @@ def test_dispatch_contract_errors(O, W):
     assert W.HydrDispatchKernel(0, 2, e0, [0, len(epi[1])], j0, [0, len(epj[1])]) == 0
     assert W.HydrRetrieveKernel(0, 2, [0, len(epi[1])], r) == 0
     assert np.array_equal(r[1], res[1])
+    # a walk with i particles and no j candidates: the functor's loop is
+    # empty (src/force.h:31-60), so acc = 0, div_v = 0, dt = C_CFL h_i / c_i;
+    # retrieve assigns every field (src/kernel.cu:392-404), stale values or not
+    lone = epi[1][:5]
+    r = [np.full(5, 7.0, W.res_dtype(2))]
+    assert W.HydrDispatchKernel(0, 1, [lone], [5], [epj[1][:0]], [0]) == 0
+    assert W.HydrRetrieveKernel(0, 1, [5], r) == 0
+    fp = W.reference_params(2)
+    assert np.all(r[0]["acc"] == 0.0) and np.all(r[0]["div_v"] == 0.0)
+    assert np.allclose(r[0]["dt"], fp.c_cfl * lone["smth"] / lone["snds"], rtol=1e-6)
     W.finalize()
 
 
