@@ -97,6 +97,30 @@ def test_large_sparse_grid(O, W, precision, tol):
     assert rel(out["div_v"], s.divv) <= tol
 
 
+def test_escaped_particles_clamped_exactly(O, W):
+    # a box that cuts the water column and side walls at y = 1.5 m: the
+    # particles above the grid are clamped into its top row of cells and
+    # counted as escaped; their cell-local coordinates keep the true offset
+    # from that cell's corner, so every distance and every chord window is
+    # still exact and the force pass matches the oracle for all particles
+    s, p, op = developed(O, W, 2, 0.02, 20)
+    s.force(op, 8)
+    lo = [float(s.x[0].min()) - 0.05, float(s.x[1].min()) - 0.05]
+    hi = [float(s.x[0].max()) + 0.05, 1.5]
+    with W.Simulation.from_state(p, state_dict(s), box_lo=lo, box_hi=hi) as sim:
+        sim.force()
+        out = sim.download()
+        st = sim.stats()
+        glo, L, nc = sim.grid()
+        pairs = int(oracle_pairs(O, sim, s).sum())
+    top = glo[1] + int(nc[1]) * L
+    outside = int(np.sum(s.x[1] >= top))
+    assert outside > 0 and st["escaped"] == outside
+    assert st["pairs_last"] == pairs
+    assert rel(out["acc"], s.a) <= 1e-5
+    assert rel(out["div_v"], s.divv) <= 1e-5
+
+
 @pytest.mark.parametrize("vsig_mode", [0, 1])
 def test_force_pass_with_variable_h_and_m(O, W, dt_neighbours, vsig_mode):
     # the general pair body (per-pair mean h), in both v_sig modes

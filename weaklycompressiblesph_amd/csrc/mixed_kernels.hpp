@@ -346,12 +346,14 @@ void k_nforce_mixed(NFMixedArgs a) {
     auto row_bounds = [&](int r, uint32_t& b0, uint32_t& b1) {
         const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
         const int srow = (dz * (int)ncy + dy) * (int)ncx;  // wave-uniform
-        const float gy = axis_gap(dy, xi[1], Lf);
-        const float gz = D == 3 ? axis_gap(dz, xi[2], Lf) : 0.f;
+        const float gy = axis_gap(dy, xi[1], Lf, c[1] + dy, (int)ncy);
+        const float gz = D == 3 ? axis_gap(dz, xi[2], Lf, c[2] + dz, (int)ncz) : 0.f;
         const float g2 = fmaf(gz, gz, gy * gy);
         const float w = __builtin_amdgcn_sqrtf(fmaxf(hc2 - g2, 0.f)) * invLf;
-        const int xa = max(max((int)floorf(xi0L - w), -R), xa_min);
-        const int xb = min(min((int)floorf(xi0L + w), R), xb_max);
+        // the row's end cells are open outward (axis_gap): a chord that
+        // reaches past one keeps it
+        const int xa = min(max(max((int)floorf(xi0L - w), -R), xa_min), xb_max);
+        const int xb = max(min(min((int)floorf(xi0L + w), R), xb_max), xa_min);
         const bool ok = live && (uint32_t)(c[1] + dy) < ncy && (D == 2 || (uint32_t)(c[2] + dz) < ncz) &&
                         g2 < hc2 && xa <= xb;
         b0 = ok ? own + srow + xa : 0u;

@@ -465,6 +465,17 @@ struct NoExtra {  // FP32 and FP64 paths: the FP32 records are all there is
 // support length), yielding the candidate index range of each non-empty
 // cell in turn, with the FP32 offset (c_i - c_j) L of that cell.
 // With R = 2 cells whose box is farther than hcut from i are skipped.
+// Gap along one axis between a cell-local coordinate x and the cell d cells
+// away (index t = c + d of nc cells). Cells span [d L, (d + 1) L) in the
+// local frame, except that the grid's first and last cells also hold every
+// particle clamped into them from outside the grid (cell_of), so they are
+// open below (t == 0) or above (t == nc - 1).
+__device__ __forceinline__ float axis_gap(int d, float x, float Lf, int t, int nc) {
+    const float lo = t <= 0 ? -3.0e38f : (float)d * Lf;
+    const float hi = t >= nc - 1 ? 3.0e38f : (float)(d + 1) * Lf;
+    return fmaxf(fmaxf(lo - x, x - hi), 0.f);
+}
+
 template <int D, int R>
 struct Stencil {
     static constexpr int S = 2 * R + 1;
@@ -484,8 +495,7 @@ struct Stencil {
                 float gap2 = 0.f;
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    const float lo = (float)dd[a] * g.Lf, hi = (float)(dd[a] + 1) * g.Lf;
-                    const float gp = fmaxf(fmaxf(lo - xl[a], xl[a] - hi), 0.f);
+                    const float gp = axis_gap(dd[a], xl[a], g.Lf, c[a] + dd[a], g.nc[a]);
                     gap2 += gp * gp;
                 }
                 if (gap2 > hcut2) continue;
@@ -771,10 +781,6 @@ struct NFArgs {
     FinalArgs fin;
 };
 
-__device__ __forceinline__ float axis_gap(int d, float x, float Lf) {
-    const float lo = (float)d * Lf, hi = (float)(d + 1) * Lf;
-    return fmaxf(fmaxf(lo - x, x - hi), 0.f);
-}
 
 // Wave-wide maximum of an unsigned value over a fully active wave, returned
 // as a wave-uniform (scalar) value: row shifts 1, 2, 4, 8 leave each 16-lane
@@ -878,14 +884,17 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
         const int cy = c[1] + dy, cz = c[2] + dz;
         b0 = b1 = 0;
         if (live && cy >= 0 && cy < a.g.nc[1] && (D == 2 || (cz >= 0 && cz < a.g.nc[2]))) {
-            const float gy = axis_gap(dy, xi[1], Lf);
-            const float gz = D == 3 ? axis_gap(dz, xi[2], Lf) : 0.f;
+            const float gy = axis_gap(dy, xi[1], Lf, cy, a.g.nc[1]);
+            const float gz = D == 3 ? axis_gap(dz, xi[2], Lf, cz, a.g.nc[2]) : 0.f;
             const float g2 = gy * gy + gz * gz;
             if (g2 < hc2) {
                 const float w = sqrtf(hc2 - g2);
                 int xa = (int)floorf((xi[0] - w) * invLf), xb = (int)floorf((xi[0] + w) * invLf);
-                xa = max(max(xa, -R), -c[0]);
-                xb = min(min(xb, R), a.g.nc[0] - 1 - c[0]);
+                // the row's end cells are open outward (axis_gap): a chord
+                // that reaches past one keeps it
+                const int kmin = -c[0], kmax = a.g.nc[0] - 1 - c[0];
+                xa = min(max(max(xa, -R), kmin), kmax);
+                xb = max(min(min(xb, R), kmax), kmin);
                 if (xa <= xb) {
                     const uint32_t row = D == 3 ? (uint32_t)((cz * a.g.nc[1] + cy) * a.g.nc[0])
                                                 : (uint32_t)(cy * a.g.nc[0]);
