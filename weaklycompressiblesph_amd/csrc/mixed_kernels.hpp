@@ -29,10 +29,10 @@
 // (v_fma_mix_f32), and div v through v_dot2_f32_f16.
 //
 // Scan: per row of cells each lane walks the records of its chord-clipped
-// window; a record with at least one neighbour half goes into the lane's
-// LDS ring queue as (record | row << 25 | hit bits << 30). The queue is
-// drained K entries per lane at a time, so every lane of the wave has work
-// for most of a drain. The neighbour set is this path's FP16 predicate
+// window; a record with at least one neighbour half goes onto the lane's
+// LDS stack as (record | row << 25 | hit bits << 30). The top K entries of
+// every lane are evaluated together, so every lane of the wave has work for
+// most of a drain. The neighbour set is this path's FP16 predicate
 // 0 < r^2 < 4 h^2: it differs from the FP32 path's only for pairs within
 // ~1e-3 h of the support edge, whose kernel weight is ~1e-6.
 #pragma once
@@ -176,8 +176,8 @@ struct NFMixedArgs {
     FinalArgs fin;
 };
 
-// Tuning constants (build-time overridable): ring queue entries per lane
-// (a power of two), entries drained per lane per drain, scan records in
+// Tuning constants (build-time overridable): stack entries per lane,
+// entries drained per lane per drain, scan records in
 // flight per lane, and the waves per SIMD the register budget is fitted to.
 #ifndef WCSPH_MIX_Q
 #define WCSPH_MIX_Q 16
@@ -195,7 +195,8 @@ constexpr int kMixQ = WCSPH_MIX_Q;
 constexpr int kMixK = WCSPH_MIX_K;
 constexpr int kMixU = WCSPH_MIX_U;
 constexpr int kMixRecBits = 25;  // record index bits of a queue entry
-static_assert((kMixQ & (kMixQ - 1)) == 0 && kMixK <= kMixQ && kMixU < kMixQ, "queue shape");
+// one drain of K entries makes room for the next U pushes
+static_assert(kMixU <= kMixK && kMixK <= kMixQ, "stack shape");
 
 // Buffer resource over n bytes at p for range-checked loads: a raw buffer
 // load whose offset lies beyond n returns zeros instead of touching memory.
@@ -212,7 +213,7 @@ void k_nforce_mixed(NFMixedArgs a) {
     constexpr int Q = kMixQ, K = kMixK, U = kMixU;
     constexpr uint32_t kRecMask = (1u << kMixRecBits) - 1u;
     static_assert(NR <= 32, "row index of a queue entry has 5 bits");
-    // slot-major ring queues: slot t of lane l at [t][l], so lanes reading
+    // slot-major stacks: slot t of lane l at [t][l], so lanes reading
     // different slots still hit different banks
     __shared__ uint32_t qbuf[kBlock / 64][Q * 64];
     // per (row, hit bits) of a queue entry: the row's (dy L/h, dz L/h) as
@@ -227,7 +228,9 @@ void k_nforce_mixed(NFMixedArgs a) {
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    uint32_t* q = &qbuf[threadIdx.x >> 6][lane];
+    // 32-bit LDS pointers, so stack moves and compares are single ops
+    using lds_u32 = __attribute__((address_space(3))) uint32_t;
+    lds_u32* const q = (lds_u32*)&qbuf[threadIdx.x >> 6][lane];
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     const bool live = i < a.n;
     const int64_t ii = live ? i : 0;
@@ -256,7 +259,12 @@ void k_nforce_mixed(NFMixedArgs a) {
     const float hc = 2.f * a.h * 1.0001f, hc2 = hc * hc;
     float accp[3] = {0.f, 0.f, 0.f}, accv[3] = {0.f, 0.f, 0.f}, divv = 0.f;
     h2 vsig2 = zero2;
-    uint32_t qh = 0, qn = 0, total = 0;
+    // the lane's stack top: the next free slot, 64 words per slot; pushes
+    // write through it and move it by one slot per kept record, so a push
+    // costs no address arithmetic
+    lds_u32* qp = q;
+    lds_u32* const qlim = q + (Q - U) * 64;  // more than this and U pushes could overflow
+    uint32_t total = 0;
     const __amdgpu_buffer_rsrc_t pos_rsrc = buffer_rsrc(a.mx.pos, (int32_t)(a.nrec * (int64_t)sizeof(PosRec16)));
 
     // Two pair interactions (the halves of record m), each kept only if its
@@ -325,14 +333,14 @@ void k_nforce_mixed(NFMixedArgs a) {
         divv = __builtin_amdgcn_fdot2(mF, drdv, divv, false);
     };
     auto drain = [&]() {
-        const uint32_t k = min(qn, (uint32_t)K);
+        // pop the top k entries; they are evaluated bottom-up
+        const uint32_t k = min((uint32_t)(qp - q) >> 6, (uint32_t)K);
         const uint32_t kmax = wave_max(k);
+        qp -= k * 64;
         for (uint32_t t = 0; t < kmax; ++t) {
-            const uint32_t e = q[((qh + t) & (Q - 1)) * 64];
+            const uint32_t e = qp[t * 64];
             body(t < k ? e : 0u);
         }
-        qh += k;
-        qn -= k;
     };
 
     // Row r of the stencil (offsets dy, dz): the index range [b0, b1) of
@@ -400,15 +408,16 @@ void k_nforce_mixed(NFMixedArgs a) {
                 if (D == 3) r2 = hfma(dz, dz, r2);
                 const bool lo = k < npair && (k != 0 || even_start) && r2.x < four;
                 const bool hi = k < hi_lim && r2.y < four;
-                // the tail slot is free (qn < Q here), so write it always and
-                // keep it only for a record with a hit
-                q[((qh + qn) & (Q - 1)) * 64] = (ebase + k) | (lo ? 1u << 30 : 0u) | (hi ? 1u << 31 : 0u);
-                qn += (lo || hi) ? 1u : 0u;
+                // the top slot is free (at most Q - U entries before this
+                // trip), so write it always and keep it only for a record
+                // with a hit
+                *qp = (ebase + k) | (lo ? 1u << 30 : 0u) | (hi ? 1u << 31 : 0u);
+                qp += (lo || hi) ? 64 : 0;
             }
-            if (__any(qn > (uint32_t)(Q - U))) drain();
+            if (__any(qp > qlim)) drain();
         }
     }
-    while (__any(qn > 0u)) drain();
+    while (__any(qp != q)) drain();
     double cand = 1.0e30;
     if (live) {
         const int64_t n = a.n;
