@@ -51,6 +51,8 @@ struct alignas(16) VelRec16 {
 struct alignas(16) StateRec16 {
     float rho[2], pr[2];
 };
+// the three records of a pair share one byte offset
+static_assert(sizeof(PosRec16) == 16 && sizeof(VelRec16) == 16 && sizeof(StateRec16) == 16, "record size");
 
 struct Mixed {
     PosRec16* pos;  // [(n + 1) / 2]
@@ -213,12 +215,12 @@ void k_nforce_mixed(NFMixedArgs a) {
     constexpr int Q = kMixQ, K = kMixK, U = kMixU;
     constexpr uint32_t kRecMask = (1u << kMixRecBits) - 1u;
     static_assert(NR <= 32, "row index of a queue entry has 5 bits");
-    // slot-major stacks: slot t of lane l at [t][l], so lanes reading
-    // different slots still hit different banks
-    __shared__ uint32_t qbuf[kBlock / 64][Q * 64];
     // per (row, hit bits) of a queue entry: the row's (dy L/h, dz L/h) as
     // half2 splats, the half2 mask keeping the hit halves, the hit count
     __shared__ uint4 etab[128];
+    // slot-major stacks: slot t of lane l at [t][l], so lanes reading
+    // different slots still hit different banks
+    __shared__ uint32_t qbuf[kBlock / 64][Q * 64];
     if (threadIdx.x < 128) {
         const int row = (int)threadIdx.x & 31, bits = (int)threadIdx.x >> 5;
         const h2 dy = hsplat((float)(row % S - R)) * a.k_Ls;
@@ -231,6 +233,11 @@ void k_nforce_mixed(NFMixedArgs a) {
     // 32-bit LDS pointers, so stack moves and compares are single ops
     using lds_u32 = __attribute__((address_space(3))) uint32_t;
     lds_u32* const q = (lds_u32*)&qbuf[threadIdx.x >> 6][lane];
+    // every slot at or above a lane's stack top holds zero or a hit-free
+    // entry (the pushes write the top slot always, the drains clear what
+    // they pop), so a drain evaluates slots above the top with no select
+#pragma unroll
+    for (int s = 0; s < Q; ++s) q[s * 64] = 0u;
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     const bool live = i < a.n;
     const int64_t ii = live ? i : 0;
@@ -265,21 +272,30 @@ void k_nforce_mixed(NFMixedArgs a) {
     lds_u32* qp = q;
     lds_u32* const qlim = q + (Q - U) * 64;  // more than this and U pushes could overflow
     uint32_t total = 0;
-    const __amdgpu_buffer_rsrc_t pos_rsrc = buffer_rsrc(a.mx.pos, (int32_t)(a.nrec * (int64_t)sizeof(PosRec16)));
+    // range-checked record loads: a hit-free entry at the stack top may name
+    // a record past the last one, which then reads as zeros
+    const int32_t rec_bytes = (int32_t)(a.nrec * (int64_t)sizeof(PosRec16));
+    const __amdgpu_buffer_rsrc_t pos_rsrc = buffer_rsrc(a.mx.pos, rec_bytes);
+    const __amdgpu_buffer_rsrc_t vel_rsrc = buffer_rsrc(a.mx.vel, rec_bytes);
+    const __amdgpu_buffer_rsrc_t st_rsrc = buffer_rsrc(a.mx.st, rec_bytes);
+    const __amdgpu_buffer_rsrc_t ch_rsrc = buffer_rsrc(a.mx.ch, (int32_t)(a.nrec * (int64_t)sizeof(h2)));
 
     // Two pair interactions (the halves of record m), each kept only if its
     // hit bit is set: the other half is then outside the window, beyond the
     // support or the particle itself. The geometry is packed FP16; the
     // state terms and the coefficients are FP32 per half. Every lane runs
-    // this for every drain step; a lane with nothing left passes e = 0 (no
-    // hit bits), which adds exactly zero.
+    // this for every drain step; an entry with no hit bits (zero, or a
+    // hit-free entry of any record) adds exactly zero. A record index past
+    // 2^25 - 1 can only carry into the row bits (row 24 -> 25 at most),
+    // never into the hit bits.
     auto body = [&](uint32_t e) {
         const uint32_t m = e & kRecMask;
         const uint4 T = etab[e >> kMixRecBits];
-        const PosRec16 P = a.mx.pos[m];
-        const VelRec16 V = a.mx.vel[m];
-        const StateRec16 St = a.mx.st[m];
-        const h2 chj = a.mx.ch[m];
+        const int32_t off = (int32_t)(m * (uint32_t)sizeof(PosRec16));
+        const PosRec16 P = __builtin_bit_cast(PosRec16, __builtin_amdgcn_raw_buffer_load_b128(pos_rsrc, off, 0, 0));
+        const VelRec16 V = __builtin_bit_cast(VelRec16, __builtin_amdgcn_raw_buffer_load_b128(vel_rsrc, off, 0, 0));
+        const StateRec16 St = __builtin_bit_cast(StateRec16, __builtin_amdgcn_raw_buffer_load_b128(st_rsrc, off, 0, 0));
+        const h2 chj = as_h2(__builtin_amdgcn_raw_buffer_load_b32(ch_rsrc, (int32_t)(m * (uint32_t)sizeof(h2)), 0, 0));
         total += T.w;
         const h2 dx = hfma(cxi2 - P.cx, Ls2, xi2 - P.x);
         const h2 dy = (yi2 - P.y) - as_h2(T.x);
@@ -333,13 +349,16 @@ void k_nforce_mixed(NFMixedArgs a) {
         divv = __builtin_amdgcn_fdot2(mF, drdv, divv, false);
     };
     auto drain = [&]() {
-        // pop the top k entries; they are evaluated bottom-up
+        // pop the top k entries and evaluate them bottom-up, clearing each
+        // slot read; a lane with k < kmax reads on into slots above its top
+        // (all below Q: its stack is then empty and kmax <= K <= Q)
         const uint32_t k = min((uint32_t)(qp - q) >> 6, (uint32_t)K);
         const uint32_t kmax = wave_max(k);
         qp -= k * 64;
         for (uint32_t t = 0; t < kmax; ++t) {
             const uint32_t e = qp[t * 64];
-            body(t < k ? e : 0u);
+            qp[t * 64] = 0u;
+            body(e);
         }
     };
 
@@ -351,12 +370,16 @@ void k_nforce_mixed(NFMixedArgs a) {
     const uint32_t own = (uint32_t)((c[2] * (int)ncy + c[1]) * (int)ncx + c[0]);  // own cell index
     const float xi0L = xi[0] * invLf;
     const int xa_min = -c[0], xb_max = (int)ncx - 1 - c[0];
+    float gz2 = 0.f;  // squared z gap of the current plane of rows
     auto row_bounds = [&](int r, uint32_t& b0, uint32_t& b1) {
         const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
         const int srow = (dz * (int)ncy + dy) * (int)ncx;  // wave-uniform
+        if (D == 3 && dy == -R) {  // first row of a plane (a wave-uniform branch)
+            const float gz = axis_gap(dz, xi[2], Lf, c[2] + dz, (int)ncz);
+            gz2 = gz * gz;
+        }
         const float gy = axis_gap(dy, xi[1], Lf, c[1] + dy, (int)ncy);
-        const float gz = D == 3 ? axis_gap(dz, xi[2], Lf, c[2] + dz, (int)ncz) : 0.f;
-        const float g2 = fmaf(gz, gz, gy * gy);
+        const float g2 = fmaf(gy, gy, gz2);
         const float w = __builtin_amdgcn_sqrtf(fmaxf(hc2 - g2, 0.f)) * invLf;
         // the row's end cells are open outward (axis_gap): a chord that
         // reaches past one keeps it
