@@ -196,7 +196,7 @@ struct NFMixedArgs {
 constexpr int kMixQ = WCSPH_MIX_Q;
 constexpr int kMixK = WCSPH_MIX_K;
 constexpr int kMixU = WCSPH_MIX_U;
-constexpr int kMixRecBits = 25;  // record index bits of a queue entry
+constexpr int kMixRecBits = 25;  // record index bits of a stack entry
 // one drain of K entries makes room for the next U pushes
 static_assert(kMixU <= kMixK && kMixK <= kMixQ, "stack shape");
 
@@ -214,8 +214,8 @@ void k_nforce_mixed(NFMixedArgs a) {
     constexpr int NR = D == 3 ? S * S : S;
     constexpr int Q = kMixQ, K = kMixK, U = kMixU;
     constexpr uint32_t kRecMask = (1u << kMixRecBits) - 1u;
-    static_assert(NR <= 32, "row index of a queue entry has 5 bits");
-    // per (row, hit bits) of a queue entry: the row's (dy L/h, dz L/h) as
+    static_assert(NR <= 32, "row index of a stack entry has 5 bits");
+    // per (row, hit bits) of a stack entry: the row's (dy L/h, dz L/h) as
     // half2 splats, the half2 mask keeping the hit halves, the hit count
     __shared__ uint4 etab[128];
     // slot-major stacks: slot t of lane l at [t][l], so lanes reading
