@@ -355,29 +355,30 @@ def test_valu_probe_rates(W):
 _DEVELOPED = {}
 
 
-def developed_3d(O, W):
-    # one developed 3D state shared by the variant cases below
-    if "s" not in _DEVELOPED:
-        _DEVELOPED["s"] = developed(O, W, 3, 0.03, 100, **SMALL3)
-    return _DEVELOPED["s"]
+def developed_state(O, W, dim):
+    # one developed state per dimension, shared by the variant cases below
+    if dim not in _DEVELOPED:
+        _DEVELOPED[dim] = developed(O, W, 3, 0.03, 100, **SMALL3) if dim == 3 else developed(O, W, 2, 0.02, 60)
+    return _DEVELOPED[dim]
 
 
 @pytest.mark.parametrize("precision,tol_acc,tol_div", [("mixed", 1e-2, 5e-3), ("fp32", 1e-5, 1e-5),
                                                         ("fp64", 1e-12, 1e-12)])
 @pytest.mark.parametrize("cps", [1, 2])
 @pytest.mark.parametrize("beta,vsig", [(0.0, 1), (0.2, 0), (0.2, 1)])
-def test_force_variants_beta_vsig_cells(O, W, dt_neighbours, precision, tol_acc, tol_div, cps, beta, vsig):
+@pytest.mark.parametrize("dim", [2, 3])
+def test_force_variants_beta_vsig_cells(O, W, dt_neighbours, dim, precision, tol_acc, tol_div, cps, beta, vsig):
     # the (cells per support, beta != 0, v_sig mode) instances of the force
-    # passes on one developed 3D state, each against the oracle with the
-    # same parameters
-    s0, p0, _ = developed_3d(O, W)
-    p = W.dam_break_params(3, beta=beta, vsig_mode=vsig)
-    op = O.sim_params(3, alpha=p.alpha, beta=beta, vsig_mode=vsig, c_cfl=p.c_cfl, c0=p.c0, g=p.g, nu=p.nu)
-    s = O.State(3, s0.n)
+    # passes on one developed state per dimension, each against the oracle
+    # with the same parameters
+    s0, p0, _ = developed_state(O, W, dim)
+    p = W.dam_break_params(dim, beta=beta, vsig_mode=vsig)
+    op = O.sim_params(dim, alpha=p.alpha, beta=beta, vsig_mode=vsig, c_cfl=p.c_cfl, c0=p.c0, g=p.g, nu=p.nu)
+    s = O.State(dim, s0.n)
     s.x[:], s.v[:], s.dens[:], s.mass[:], s.h[:], s.type[:] = s0.x, s0.v, s0.dens, s0.mass, s0.h, s0.type
     s.eos(op)
     s.force(op, 8)
-    print(f"case {precision} cps={cps} beta={beta} vsig={vsig}", flush=True)
+    print(f"case {dim}D {precision} cps={cps} beta={beta} vsig={vsig}", flush=True)
     with W.Simulation.from_state(p, state_dict(s), precision=precision, cells_per_support=cps) as sim:
         sim.force()
         m = sim.download()
