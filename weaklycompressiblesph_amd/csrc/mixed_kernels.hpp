@@ -178,14 +178,23 @@ struct NFMixedArgs {
     FinalArgs fin;
 };
 
-// Tuning constants (build-time overridable): stack entries per lane,
-// entries drained per lane per drain, scan records in
-// flight per lane, and the waves per SIMD the register budget is fitted to.
+// Tuning constants (build-time overridable): stack entries per lane and
+// entries drained per lane per drain (3D, and 2D after the 2), scan
+// records in flight per lane, and the waves per SIMD the register budget
+// is fitted to. In 3D, 20 entries keep a 256-thread block at 22.5 KB of
+// LDS, so the 7 blocks per CU its registers allow still fit in 160 KB; the
+// 2D kernel's registers allow 8, which 16 entries (18.5 KB) keep.
 #ifndef WCSPH_MIX_Q
-#define WCSPH_MIX_Q 16
+#define WCSPH_MIX_Q 20
 #endif
 #ifndef WCSPH_MIX_K
-#define WCSPH_MIX_K 8
+#define WCSPH_MIX_K 10
+#endif
+#ifndef WCSPH_MIX_Q2
+#define WCSPH_MIX_Q2 16
+#endif
+#ifndef WCSPH_MIX_K2
+#define WCSPH_MIX_K2 8
 #endif
 #ifndef WCSPH_MIX_U
 #define WCSPH_MIX_U 2
@@ -195,10 +204,12 @@ struct NFMixedArgs {
 #endif
 constexpr int kMixQ = WCSPH_MIX_Q;
 constexpr int kMixK = WCSPH_MIX_K;
+constexpr int kMixQ2 = WCSPH_MIX_Q2;
+constexpr int kMixK2 = WCSPH_MIX_K2;
 constexpr int kMixU = WCSPH_MIX_U;
 constexpr int kMixRecBits = 25;  // record index bits of a stack entry
 // one drain of K entries makes room for the next U pushes
-static_assert(kMixU <= kMixK && kMixK <= kMixQ, "stack shape");
+static_assert(kMixU <= kMixK && kMixK <= kMixQ && kMixU <= kMixK2 && kMixK2 <= kMixQ2, "stack shape");
 
 // Buffer resource over n bytes at p for range-checked loads: a raw buffer
 // load whose offset lies beyond n returns zeros instead of touching memory.
@@ -212,7 +223,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WCSPH_MI
 void k_nforce_mixed(NFMixedArgs a) {
     constexpr int S = 2 * R + 1;
     constexpr int NR = D == 3 ? S * S : S;
-    constexpr int Q = kMixQ, K = kMixK, U = kMixU;
+    constexpr int Q = D == 3 ? kMixQ : kMixQ2, K = D == 3 ? kMixK : kMixK2, U = kMixU;
     constexpr uint32_t kRecMask = (1u << kMixRecBits) - 1u;
     static_assert(NR <= 32, "row index of a stack entry has 5 bits");
     // per (row, hit bits) of a stack entry: the row's (dy L/h, dz L/h) as
