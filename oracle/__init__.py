@@ -74,6 +74,9 @@ def lib():
         L.orc_neighbours_brute_cells.restype = C.c_int
         L.orc_neighbours_brute_cells.argtypes = [C.c_int, C.c_int64, f64p, f32p, f64p, C.c_double, i32p, i32p,
                                                  C.c_void_p, C.c_int]
+        L.orc_neighbour_counts_cells.restype = C.c_int64
+        L.orc_neighbour_counts_cells.argtypes = [C.c_int, C.c_int64, f64p, f32p, f64p, C.c_double, i32p, i32p,
+                                                 C.c_int]
         L.orc_make_walks.restype = C.c_int64
         L.orc_make_walks.argtypes = [C.c_int, C.c_int64, f64p, C.c_double, C.c_int,
                                      C.POINTER(C.c_int64), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
@@ -179,6 +182,20 @@ def neighbours_brute_cells(pos, h, lo, L, nc, cap=0):
                                      float(L), np.ascontiguousarray(nc[:dim], np.int32), counts,
                                      lists.ctypes.data if cap else None, cap)
     return (counts, lists) if cap else counts
+
+
+def neighbour_counts_cells(pos, h, lo, L, nc, nthreads=1):
+    """Counts of the cell-local predicate through a cell list (large states;
+    every particle inside the grid, else ValueError)."""
+    dim, n = pos.shape
+    counts = np.zeros(n, np.int32)
+    tot = lib().orc_neighbour_counts_cells(dim, n, np.ascontiguousarray(pos, np.float64),
+                                           np.ascontiguousarray(h, np.float32),
+                                           np.ascontiguousarray(lo[:dim], np.float64), float(L),
+                                           np.ascontiguousarray(nc[:dim], np.int32), counts, nthreads)
+    if tot < 0:
+        raise ValueError("a particle lies outside the grid")
+    return counts
 
 
 def make_walks(pos, r_search, group_max=128):

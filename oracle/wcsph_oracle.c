@@ -250,6 +250,87 @@ int orc_neighbours_brute_cells(int dim, int64_t n, const double* pos, const floa
     return mx;
 }
 
+/* The cell-local predicate of orc_neighbours_brute_cells, counted through a
+ * cell list over the same grid, for states too large for the O(N^2) pass:
+ * a pair passing the predicate has r < H <= 2 h_max, so it lies within
+ * ceil(2 h_max / L) cells along each axis. Every particle must lie inside
+ * the grid (a clamped one keeps its true offset from the edge cell, which
+ * this window does not follow): returns -1 if one does not, else the sum
+ * of the counts. */
+int64_t orc_neighbour_counts_cells(int dim, int64_t n, const double* pos, const float* h,
+                                   const double* lo, double L, const int* nc, int* counts, int nthreads) {
+    int* ci = (int*)malloc(sizeof(int) * 3 * (size_t)n);
+    float* xl = (float*)malloc(sizeof(float) * 3 * (size_t)n);
+    float hmax = 0.0f;
+    int outside = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        double x[3];
+        gather_pos(dim, n, pos, i, x);
+        cell_local(dim, x, lo, L, nc, ci + 3 * i, xl + 3 * i);
+        for (int k = 0; k < dim; ++k)
+            if (!(x[k] >= lo[k] && x[k] < lo[k] + nc[k] * L)) outside = 1;
+        if (h[i] > hmax) hmax = h[i];
+    }
+    if (outside) {
+        free(ci); free(xl);
+        return -1;
+    }
+    int64_t ncell = 1;
+    for (int k = 0; k < dim; ++k) ncell *= nc[k];
+    int64_t* cstart = (int64_t*)calloc((size_t)ncell + 1, sizeof(int64_t));
+    int64_t* cidx = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)ncell);
+    for (int64_t i = 0; i < n; ++i) {
+        const int* c = ci + 3 * i;
+        const int64_t cc = dim == 3 ? ((int64_t)c[2] * nc[1] + c[1]) * nc[0] + c[0] : (int64_t)c[1] * nc[0] + c[0];
+        cstart[cc + 1]++;
+    }
+    for (int64_t c = 0; c < ncell; ++c) cstart[c + 1] += cstart[c];
+    memcpy(fill, cstart, sizeof(int64_t) * (size_t)ncell);
+    for (int64_t i = 0; i < n; ++i) {
+        const int* c = ci + 3 * i;
+        const int64_t cc = dim == 3 ? ((int64_t)c[2] * nc[1] + c[1]) * nc[0] + c[0] : (int64_t)c[1] * nc[0] + c[0];
+        cidx[fill[cc]++] = i;
+    }
+    const int R = (int)ceil(2.0 * (double)hmax / L);
+    const float Lf = (float)L;
+    int64_t total = 0;
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : total) num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+        const int* c = ci + 3 * i;
+        int cnt = 0;
+        const int zr = dim == 3 ? R : 0;
+        for (int dz = -zr; dz <= zr; ++dz)
+            for (int dy = -R; dy <= R; ++dy)
+                for (int dx = -R; dx <= R; ++dx) {
+                    const int cx = c[0] + dx, cy = c[1] + dy, cz = (dim == 3 ? c[2] : 0) + dz;
+                    if (cx < 0 || cy < 0 || cx >= nc[0] || cy >= nc[1]) continue;
+                    if (dim == 3 && (cz < 0 || cz >= nc[2])) continue;
+                    const int64_t cc = dim == 3 ? ((int64_t)cz * nc[1] + cy) * nc[0] + cx : (int64_t)cy * nc[0] + cx;
+                    for (int64_t t = cstart[cc]; t < cstart[cc + 1]; ++t) {
+                        const int64_t j = cidx[t];
+                        if (j == i) continue;
+                        float d[3] = {0.0f, 0.0f, 0.0f};
+                        for (int k = 0; k < dim; ++k)
+                            d[k] = fmaf((float)(ci[3 * i + k] - ci[3 * j + k]), Lf, xl[3 * i + k]) - xl[3 * j + k];
+                        float r2 = d[0] * d[0];
+                        r2 = fmaf(d[1], d[1], r2);
+                        if (dim == 3) r2 = fmaf(d[2], d[2], r2);
+                        const float H = h[i] + h[j];
+                        if (r2 > 0.0f && r2 < H * H) ++cnt;
+                    }
+                }
+        counts[i] = cnt;
+        total += cnt;
+    }
+    free(ci); free(xl); free(cstart); free(cidx); free(fill);
+    (void)nthreads;
+    return total;
+}
+
 /* ------------------------------------------------------------------ */
 /* Fake FDPS multiwalk grouping                                        */
 /* ------------------------------------------------------------------ */

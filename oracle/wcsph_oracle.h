@@ -81,6 +81,12 @@ int orc_neighbours_brute_cells(int dim, int64_t n, const double* pos, const floa
                                const double* lo, double L, const int* nc,
                                int* counts, int* lists, int cap);
 
+/* The same predicate and counts as orc_neighbours_brute_cells, through a
+ * cell list over the grid (for large states); every particle must lie
+ * inside the grid: returns -1 if one does not, else the summed counts. */
+int64_t orc_neighbour_counts_cells(int dim, int64_t n, const double* pos, const float* h,
+                                   const double* lo, double L, const int* nc, int* counts, int nthreads);
+
 /* Fake FDPS multiwalk grouping: Morton-ordered i-groups of at most
  * group_max particles, j list = every particle within r_search of the
  * group's bounding box (a superset of each i's support, as FDPS Symmetry

@@ -241,6 +241,91 @@ def test_lattice_interior_gradients(O, dim):
     assert np.allclose(res["acc"][0], -gradP / 1000.0, rtol=0.05, atol=0.05 * np.abs(gradP).max() / 1000)
 
 
+def shear_lattice(O, dim, field, *, nu_i, nu_j, rho_i=1000.0, rho_j=1000.0, dx=0.02):
+    """Lattice block centred on its middle particle, P = 0 everywhere, with
+    velocity field(g) (g: (n, dim) positions relative to the centre); returns
+    (i record of the centre, j records of the whole block)."""
+    m = 13 if dim == 3 else 31
+    g = np.stack(np.meshgrid(*[np.arange(m)] * dim, indexing="ij"), -1).reshape(-1, dim) * dx
+    c = int(np.argmin(((g - g.mean(0)) ** 2).sum(1)))
+    g = g - g[c]
+    epj = np.zeros(len(g), O.epj_dtype(dim))
+    epj["pos"], epj["vel"] = g, field(g)
+    epj["smth"], epj["dens"], epj["snds"], epj["visc"] = 1.3 * dx, rho_j, 30.0, nu_j
+    epj["mass"] = rho_j * dx ** dim  # particle volume dx^D
+    epi = np.zeros(1, O.epi_dtype(dim))
+    for k in ("pos", "vel", "smth", "snds"):
+        epi[k] = epj[k][c]
+    epi["dens"], epi["visc"] = rho_i, nu_i
+    return epi, epj
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("nu_i,nu_j,rho_j", [(1e-2, 1e-2, 1000.0), (1e-2, 3e-2, 1000.0), (1e-2, 1e-2, 3000.0)])
+def test_laminar_viscosity_shear_closed_form(O, dim, nu_i, nu_j, rho_j):
+    # Morris laminar term (src/force.h:53) in closed form. The SPH Laplacian
+    #   lap v_i ~ 2 sum_j V_j (v_i - v_j) (r_ij . gradW_ij) / r_ij^2
+    # (V_j = m_j / rho_j) is exact for quadratic fields on a symmetric
+    # lattice up to the kernel's second moment (1.5% at h = 1.3 dx in 3D)
+    # and the 0.01 h^2 regulariser. The functor's coefficient
+    # m_j 2 (nu_i + nu_j) / (rho_i + rho_j) then gives, for v_x = A y^2
+    # (lap v_x = 2 A) with uniform nu_j and rho_j around i:
+    #   a_x = (nu_i + nu_j)/2 * 2 rho_j / (rho_i + rho_j) * 2 A
+    # which is nu lap v for one fluid. A coefficient halved or doubled, nu_i
+    # in place of the mean, or 2 rho_i in place of rho_i + rho_j fails.
+    A = 2.5
+    def field(g):
+        v = np.zeros_like(g)
+        v[:, 0] = A * g[:, 1] ** 2
+        return v
+    epi, epj = shear_lattice(O, dim, field, nu_i=nu_i, nu_j=nu_j, rho_j=rho_j)
+    res = O.hydro_force(O.force_params(dim, alpha=0.0), epi, epj)
+    expect = 0.5 * (nu_i + nu_j) * 2.0 * rho_j / (1000.0 + rho_j) * 2.0 * A
+    assert res["acc"][0][0] == pytest.approx(expect, rel=0.03)
+    assert np.all(np.abs(res["acc"][0][1:]) < 1e-9 * expect)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_laminar_viscosity_linear_field_is_zero(O, dim):
+    # a linear velocity field has no Laplacian: on the symmetric lattice the
+    # laminar term cancels pair by pair (j and its mirror image), so the
+    # acceleration is zero to rounding while each pair term is not
+    G = np.random.default_rng(12).normal(size=(dim, dim))
+    epi, epj = shear_lattice(O, dim, lambda g: g @ G.T, nu_i=1e-2, nu_j=1e-2)
+    res = O.hydro_force(O.force_params(dim, alpha=0.0), epi, epj)
+    one = O.hydro_force(O.force_params(dim, alpha=0.0), epi, epj[np.argsort(
+        ((epj["pos"] - np.array([0.02] + [0.0] * (dim - 1))) ** 2).sum(1))[:1]])
+    assert np.abs(one["acc"][0]).max() > 1e-3  # one neighbour alone does act
+    assert np.abs(res["acc"][0]).max() < 1e-12 * np.abs(one["acc"][0]).max() * len(epj)
+
+
+def test_internal_energy_of_tait_law(O):
+    # the internal energy summed by the conservation diagnostics is the
+    # work of compression of the clamped Tait law: e(rho0) = 0,
+    # de/drho = P/rho^2 (P from the separately pinned EoS), and e = 0 below
+    # rho0 where P is clamped to 0. A wrong constant in e fails e(rho0) = 0
+    # or the derivative.
+    p = O.sim_params(3)
+    s = O.State(3, 1)
+    s.mass[0], s.h[0] = 2.0, 0.01
+
+    def e(rho):
+        s.dens[0] = rho
+        return s.conservation(p)[6] / s.mass[0]
+
+    assert e(p.rho0) == 0.0
+    assert e(0.97 * p.rho0) == 0.0 and e(0.5 * p.rho0) == 0.0
+    for rho in (1000.2, 1004.0, 1030.0, 1150.0):
+        d = 1e-3 * (rho - p.rho0)
+        dedrho = (e(rho + d) - e(rho - d)) / (2 * d)
+        P = O.eos_pressure(rho, p.rho0, p.c0, p.gamma)
+        assert dedrho == pytest.approx(P / rho ** 2, rel=1e-6)
+    # and e integrates P/rho^2 from rho0 (trapezoid on a fine grid)
+    rr = np.linspace(p.rho0, 1050.0, 20001)
+    f = np.array([O.eos_pressure(r, p.rho0, p.c0, p.gamma) for r in rr]) / rr ** 2
+    assert e(1050.0) == pytest.approx(np.sum(0.5 * (f[1:] + f[:-1]) * np.diff(rr)), rel=1e-7)
+
+
 # --------------------------------------------------------------------------
 # neighbour sets, FDPS-style walks
 # --------------------------------------------------------------------------
@@ -305,6 +390,26 @@ def test_free_fall_is_exact(O):
     assert s.x[2, 0] == pytest.approx(5.0 - 0.5 * p.g * t * t, rel=1e-12)
     assert s.v[2, 0] == pytest.approx(-p.g * t, rel=1e-12)
     assert s.x[0, 0] == 0.3 and s.v[0, 0] == 0.0
+
+
+def test_viscous_time_step_bound(O):
+    # getTimeStepGlobal (src/integral.h:2-12) with a viscosity large enough
+    # that the explicit-diffusion bound dt <= 0.125 h^2 / nu (src/integral.h:8)
+    # is the binding one: one isolated particle, h = 0.01 m, nu = 1 m^2/s gives
+    # 1.25e-5 s against C h / c = 4.0e-5 s and 0.25 sqrt(h / g) = 7.98e-3 s
+    s = O.State(3, 1)
+    s.x[:, 0] = [0.3, 0.2, 5.0]
+    s.dens[0], s.mass[0], s.h[0] = 1000.0, 1e-3, 0.01
+    p = O.sim_params(3, nu=1.0)
+    s.init(p)
+    assert s.global_dt(p) == pytest.approx(1.25e-5, rel=1e-14)
+    # below that viscosity the CFL bound of the particle takes over again
+    q = O.sim_params(3, nu=0.1)
+    s.init(q)
+    assert s.global_dt(q) == pytest.approx(0.25 * 0.01 / q.c0, rel=1e-14)
+    # walls never bound the step
+    s.type[0] = 1
+    assert s.global_dt(p) == 1.0e30
 
 
 def test_walls_do_not_move(O):
@@ -552,3 +657,21 @@ def test_step_composition_with_shepard(O):
     dt2 = s.global_dt(p)
     s.step(p)
     assert np.allclose(s.dens, rho2 - rho2 * dt2 * dv2, rtol=1e-14)
+
+
+@pytest.mark.parametrize("dim,R", [(2, 1), (2, 2), (3, 1), (3, 2)])
+def test_neighbour_counts_cells_equal_brute_force(O, dim, R):
+    # the cell-list count of the cell-local predicate (for large states)
+    # equals the O(N^2) pass over the same grid, particle by particle, with
+    # varying h and cells of 2 h_max / R
+    n = 1500 if dim == 2 else 2500
+    pos = RNG.uniform(0, 1, (dim, n))
+    h = ((0.03 if dim == 2 else 0.07) * (1 + 0.3 * RNG.uniform(-1, 1, n))).astype(np.float32)
+    L = 2 * float(h.max()) * (1 + 1e-5) / R
+    lo = np.full(3, -0.05)
+    nc = np.array([int(np.ceil(1.1 / L))] * 3, np.int32)
+    ref = O.neighbours_brute_cells(pos, h, lo, L, nc)
+    got = O.neighbour_counts_cells(pos, h, lo, L, nc, 4)
+    assert np.array_equal(got, ref) and ref.sum() > 10 * n
+    with pytest.raises(ValueError):
+        O.neighbour_counts_cells(pos + 2.0, h, lo, L, nc)
