@@ -38,6 +38,8 @@ extern "C" {
 #define WCSPH_ENOMEM 4      /* allocation failed or over a stated budget */
 #define WCSPH_EOVERFLOW 5   /* neighbour list or cell grid capacity exceeded */
 #define WCSPH_EIO 6         /* a file could not be opened, read, written or parsed */
+#define WCSPH_EDIVERGED 7   /* a simulation's state has diverged: a non-finite position,
+                               or more particles in one cell than the cell list ranks */
 
 /* ---------------- FDPS boundary records (64-bit FDPS build) ----------------
  * 3D sizes: EPI 96 B, EPJ 112 B, RESULT 40 B; 2D: 80 / 96 / 32 B. */
@@ -149,6 +151,9 @@ typedef struct {
     int64_t escaped;        /* particles clamped into the grid edge at the last binning */
     int32_t max_count;      /* largest neighbour count of the last pass */
     int32_t overflow;       /* nonzero if a neighbour list was truncated */
+    int32_t nonfinite;      /* particles found with a non-finite position (sticky; the
+                               step and sync calls then return WCSPH_EDIVERGED) */
+    int32_t overfull;       /* nonzero once a cell held more than 256 particles (sticky) */
 } wcsph_sim_stats;
 
 /* Dam-break initial condition on a square/cubic lattice (the geometry of the
@@ -179,7 +184,8 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
 int wcsph_sim_destroy(wcsph_sim* s);
 /* Upload a state (SoA, host): pos/vel [dim][n], dens/mass/h [n], type [n]
  * (0 fluid, 1 wall = FREEZE of src/class.h:33-36). acc, div_v and the
- * per-particle dt are then produced by wcsph_sim_init. */
+ * per-particle dt are then produced by wcsph_sim_init. Every value must be
+ * finite (WCSPH_EINVAL otherwise). */
 int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const double* dens,
                      const double* mass, const double* h, const int32_t* type);
 /* Pressure, first force pass, the Shepard filter when shepard_interval > 0,
@@ -188,20 +194,25 @@ int wcsph_sim_init(wcsph_sim* s);
 /* nsteps steps of src/main.cpp:89-133, queued on the simulation's stream;
  * returns without waiting. The Shepard filter runs after the drift of every
  * step whose index (0 after init) is a multiple of shepard_interval
- * (src/main.cpp:105, src/param.h:12). */
+ * (src/main.cpp:105, src/param.h:12). Returns WCSPH_EDIVERGED, queueing
+ * nothing, once steps queued by an earlier call (and completed) found a
+ * non-finite position or an overfull cell; such steps stop the clock and do
+ * no force work, so a diverged run cannot stall the device. */
 int wcsph_sim_step(wcsph_sim* s, int32_t nsteps);
 /* Steps are replayed from a captured two-step hipGraph (one submission per
  * two steps) unless per-step timing is on or graphs are disabled here. No
  * counterpart in the reference. */
 int wcsph_sim_set_graphs(wcsph_sim* s, int32_t enable);
-/* Non-blocking: *idle = 1 when all queued work of the simulation is done.
- * No counterpart in the reference. */
+/* Non-blocking: *idle = 1 when all queued work of the simulation is done
+ * (then WCSPH_EDIVERGED if the state has diverged). No counterpart in the
+ * reference. */
 int wcsph_sim_query(wcsph_sim* s, int32_t* idle);
 /* Re-bin and run one force pass on the current state (no integration):
  * acc (hydro only, no gravity), div_v and dt_i of the current positions --
  * CalcPressure and calcForceAllAndWriteBack of src/main.cpp:70-75. */
 int wcsph_sim_force(wcsph_sim* s);
-/* Wait for the simulation's queued work. No counterpart in the reference. */
+/* Wait for the simulation's queued work; WCSPH_EDIVERGED if the state has
+ * diverged. No counterpart in the reference. */
 int wcsph_sim_sync(wcsph_sim* s);
 /* Clock, pair counts and bounds of the last pass; the time and step are the
  * reference's sysinfo (src/main.cpp:89-96). */

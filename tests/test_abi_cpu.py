@@ -150,3 +150,14 @@ def test_host_worker_pool(tmp_path):
                     "-pthread"], check=True, capture_output=True)
     out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120)
     assert "pool checks: ok" in out.stdout
+
+
+def test_cell_order_ranking_host_replay():
+    # the ranked gather of k_reorder (ranked_member, compiled for the host by
+    # hipcc) puts every cell of 1..256 members into ascending order of the
+    # previous index, whatever order the scatter left it in; cells beyond
+    # 256 are flagged overfull instead of ranked (the search is quadratic)
+    exe = os.path.join(ROOT, "tests", "cpp", "rank_check")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and out.stdout.startswith("ok"), out.stdout + out.stderr

@@ -11,7 +11,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libwcsph.so")
 
-WCSPH_OK, WCSPH_EINVAL, WCSPH_EHIP, WCSPH_ESTATE, WCSPH_ENOMEM, WCSPH_EOVERFLOW, WCSPH_EIO = 0, 1, 2, 3, 4, 5, 6
+WCSPH_OK, WCSPH_EINVAL, WCSPH_EHIP, WCSPH_ESTATE, WCSPH_ENOMEM, WCSPH_EOVERFLOW, WCSPH_EIO, WCSPH_EDIVERGED = \
+    0, 1, 2, 3, 4, 5, 6, 7
 PREC_FP32, PREC_MIXED, PREC_FP64 = 0, 1, 2
 PRECISIONS = {"fp32": PREC_FP32, "mixed": PREC_MIXED, "fp64": PREC_FP64}
 
@@ -40,7 +41,8 @@ class SimStats(C.Structure):
     """wcsph_sim_stats."""
     _fields_ = [("time", C.c_double), ("dt", C.c_double), ("step", C.c_int64), ("n", C.c_int64),
                 ("pairs_last", C.c_int64), ("pairs_total", C.c_int64), ("escaped", C.c_int64),
-                ("max_count", C.c_int32), ("overflow", C.c_int32)]
+                ("max_count", C.c_int32), ("overflow", C.c_int32), ("nonfinite", C.c_int32),
+                ("overfull", C.c_int32)]
 
 
 class DamBreakDesc(C.Structure):

@@ -307,7 +307,7 @@ void dispatch(int32_t tag, int32_t n_walk, const EPI* const* epi, const int32_t*
         const int64_t tot = ni + nj;
         int c = 1;
         for (int w = 0; w < n_walk && c < nch; ++w)
-            if ((ioff[w] + joff[w]) * nch >= tot * c) wb[c++] = w;
+            if (((int64_t)ioff[w] + joff[w]) * nch >= tot * c) wb[c++] = w;
         for (; c < nch; ++c) wb[c] = n_walk;
     }
     std::vector<Layout> Ls(nch);

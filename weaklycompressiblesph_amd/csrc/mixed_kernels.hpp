@@ -250,7 +250,8 @@ void k_nforce_mixed(NFMixedArgs a) {
 #pragma unroll
     for (int s = 0; s < Q; ++s) q[s * 64] = 0u;
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-    const bool live = i < a.n;
+    const bool failed = sim_failed(a.s);
+    const bool live = i < a.n && !failed;
     const int64_t ii = live ? i : 0;
     const float4 xf = a.px[ii];
     const float xi[3] = {xf.x, xf.y, xf.z};
@@ -452,7 +453,7 @@ void k_nforce_mixed(NFMixedArgs a) {
         }
     }
     while (__any(qp != q)) drain();
-    double cand = 1.0e30;
+    double cand = failed ? 0.0 : 1.0e30;  // a diverged state stops the clock
     if (live) {
         const int64_t n = a.n;
         const float4 qi4 = a.pq[i];  // FP32 rho, P/rho^2, c, h

@@ -69,6 +69,9 @@ struct wcsph_sim {
     std::vector<hipEvent_t> tev;  // 5 events per timed step, pooled
     int64_t tsteps = 0;
     std::vector<void*> allocs;
+    // pinned copy of the device failure flags (non-finite positions, overfull
+    // cells), refreshed after every call that queues steps
+    volatile uint32_t* hflags = nullptr;
 
     template <class T>
     T* alloc(int64_t count) {
@@ -86,6 +89,7 @@ struct wcsph_sim {
         for (auto& g : graph2)
             if (g) (void)hipGraphExecDestroy(g);
         if (st) (void)hipStreamDestroy(st);
+        if (hflags) (void)hipHostFree((void*)hflags);
     }
     int64_t nb(int64_t items) const { return ceil_div(items, kBlock); }
 };
@@ -107,6 +111,24 @@ int64_t sim_bytes(int D, int64_t n, int K, int64_t ncell_guess) {
 }
 
 }  // namespace
+
+// Copy the device failure flags to the pinned mirror behind the queued work.
+static void queue_flags(wcsph_sim* s) {
+    WCSPH_HIP(hipMemcpyAsync((void*)s->hflags, &s->sc->nonfinite, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             s->st));
+}
+
+// Report a diverged state: from the mirror as of the last completed copy
+// (callers that need it current synchronise first).
+static void raise_if_failed(const wcsph_sim* s) {
+    const uint32_t nonfinite = s->hflags[0], overfull = s->hflags[1];
+    if (nonfinite == 0 && overfull == 0) return;
+    std::string msg = "the simulation has diverged:";
+    if (nonfinite) msg += " " + std::to_string(nonfinite) + " particles with a non-finite position at the last binning;";
+    if (overfull) msg += " a cell holds more than " + std::to_string(kRankCap) + " particles;";
+    msg += " steps since then have stopped the clock and done no force work";
+    throw Error{WCSPH_EDIVERGED, msg};
+}
 
 // Largest smoothing length (as FP32, the precision of the predicate) and
 // whether all are equal.
@@ -133,12 +155,14 @@ static void alloc_grid(wcsph_sim* s, const double* lo, double L, const int32_t* 
     }
     WCSPH_REQUIRE(ncell < (int64_t)1 << 31, WCSPH_EOVERFLOW, "cell grid exceeds 2^31 cells");
     s->ncell = ncell;
-    s->npart = ceil_div(ncell, kScanTile);
-    s->count = s->alloc<uint32_t>(ncell);
-    s->start = s->alloc<uint32_t>(ncell + 1);
+    // counts and starts carry one bucket past the last cell: the dead
+    // bucket of particles whose position is not finite
+    s->npart = ceil_div(ncell + 1, kScanTile);
+    s->count = s->alloc<uint32_t>(ncell + 1);
+    s->start = s->alloc<uint32_t>(ncell + 2);
     s->part = s->alloc<uint32_t>(s->npart);
     // the scan clears count after reading it, ready for the next pass
-    WCSPH_HIP(hipMemsetAsync(s->count, 0, sizeof(uint32_t) * ncell, s->st));
+    WCSPH_HIP(hipMemsetAsync(s->count, 0, sizeof(uint32_t) * (ncell + 1), s->st));
 }
 
 static void build_grid(wcsph_sim* s, const double* pos, const double* h) {
@@ -204,9 +228,10 @@ static void bin_and_reorder(wcsph_sim* s, bool step) {
     else
         hipLaunchKernelGGL(k_bin<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src, s->acc, s->divv, kick, n, s->grid,
                            s->key, s->slot, s->count, s->sc);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell + 1, s->part);
     hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanTilesThreads), 0, s->st, s->part, s->npart);
-    hipLaunchKernelGGL(k_scan_apply, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell, s->part, s->start);
+    hipLaunchKernelGGL(k_scan_apply, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell + 1, s->part,
+                       s->start);
     hipLaunchKernelGGL(k_scatter, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->key, s->slot, s->start, s->perm,
                        s->crange, n);
     const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
@@ -270,7 +295,7 @@ template <int D, int R>
 static void launch_neigh(wcsph_sim* s, int count_total) {
     ensure_lists(s);
     hipLaunchKernelGGL((k_neigh<D, R>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, s->pk, s->start, s->grid, s->n,
-                       s->K, s->hmax, s->nbr, s->cnt, s->P);
+                       s->K, s->hmax, s->nbr, s->cnt, s->P, s->sc);
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc, count_total ? 5 : 1);
 }
 
@@ -439,10 +464,10 @@ static void shepard_pass(wcsph_sim* s) {
     if (s->D == 3) {
         if (s->R == 1)
             hipLaunchKernelGGL((k_shepard<3, 1>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
-                               n, cnorm, s->rho_tmp);
+                               n, cnorm, s->rho_tmp, s->sc);
         else
             hipLaunchKernelGGL((k_shepard<3, 2>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
-                               n, cnorm, s->rho_tmp);
+                               n, cnorm, s->rho_tmp, s->sc);
         if (mixed)
             hipLaunchKernelGGL((k_apply_density<3, MixedExtra<3>>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk,
                                s->rho_tmp, s->pr64, s->c64, e, MixedExtra<3>{s->mx, s->mc}, n);
@@ -452,10 +477,10 @@ static void shepard_pass(wcsph_sim* s) {
     } else {
         if (s->R == 1)
             hipLaunchKernelGGL((k_shepard<2, 1>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
-                               n, cnorm, s->rho_tmp);
+                               n, cnorm, s->rho_tmp, s->sc);
         else
             hipLaunchKernelGGL((k_shepard<2, 2>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
-                               n, cnorm, s->rho_tmp);
+                               n, cnorm, s->rho_tmp, s->sc);
         if (mixed)
             hipLaunchKernelGGL((k_apply_density<2, MixedExtra<2>>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk,
                                s->rho_tmp, s->pr64, s->c64, e, MixedExtra<2>{s->mx, s->mc}, n);
@@ -631,6 +656,8 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         WCSPH_HIP(hipMemsetAsync(s->acc, 0, sizeof(double) * D * n, s->st));
         WCSPH_HIP(hipMemsetAsync(s->divv, 0, sizeof(double) * n, s->st));
         WCSPH_HIP(hipMemsetAsync(s->sc, 0, sizeof(Scalars), s->st));
+        WCSPH_HIP(hipHostMalloc((void**)&s->hflags, 2 * sizeof(uint32_t), hipHostMallocDefault));
+        s->hflags[0] = s->hflags[1] = 0u;
         WCSPH_HIP(hipStreamSynchronize(s->st));
         *out = s.release();
     });
@@ -651,6 +678,11 @@ int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const d
         for (int64_t i = 0; i < n; ++i) {
             WCSPH_REQUIRE(dens[i] > 0 && mass[i] > 0 && h[i] > 0, WCSPH_EINVAL,
                           "density, mass and smoothing length must be positive");
+            WCSPH_REQUIRE(std::isfinite(dens[i]) && std::isfinite(mass[i]) && std::isfinite(h[i]), WCSPH_EINVAL,
+                          "density, mass and smoothing length must be finite");
+            for (int d = 0; d < D; ++d)
+                WCSPH_REQUIRE(std::isfinite(pos[d * n + i]) && std::isfinite(vel[d * n + i]), WCSPH_EINVAL,
+                              "positions and velocities must be finite");
             WCSPH_REQUIRE(type[i] == 0 || type[i] == 1, WCSPH_EINVAL, "type must be 0 (fluid) or 1 (wall)");
         }
         build_grid(s, pos, h);
@@ -683,6 +715,7 @@ int wcsph_sim_init(wcsph_sim* s) {
         neigh_force(s, nullptr, nullptr, nullptr);
         if (s->p.shepard_interval > 0) shepard_pass(s);  // after the first force (src/main.cpp:75)
         final_stage(s, 0);
+        queue_flags(s);
         s->hstep = 0;
         s->steppable = true;
         WCSPH_HIP(hipGetLastError());
@@ -716,6 +749,7 @@ int wcsph_sim_step(wcsph_sim* s, int32_t nsteps) {
         WCSPH_REQUIRE(s && s->uploaded && s->steppable, WCSPH_ESTATE,
                       "call wcsph_sim_init first (and again after wcsph_sim_force)");
         WCSPH_REQUIRE(nsteps >= 0, WCSPH_EINVAL, "nsteps < 0");
+        raise_if_failed(s);
         WCSPH_HIP(hipSetDevice(s->device));
         // pairs of steps without a filter replay the captured graph
         for (int k = 0; k < nsteps;) {
@@ -730,6 +764,7 @@ int wcsph_sim_step(wcsph_sim* s, int32_t nsteps) {
                 ++k;
             }
         }
+        queue_flags(s);
         WCSPH_HIP(hipGetLastError());
     });
 }
@@ -751,6 +786,7 @@ int wcsph_sim_query(wcsph_sim* s, int32_t* idle) {
         }
         WCSPH_HIP(e);
         *idle = 1;
+        raise_if_failed(s);
     });
 }
 
@@ -763,6 +799,7 @@ int wcsph_sim_force(wcsph_sim* s) {
         WCSPH_HIP(hipEventRecord(s->ev[0], s->st));
         bin_and_reorder(s, false);
         neigh_force(s, s->ev[1], s->ev[2], s->ev[3]);
+        queue_flags(s);
         WCSPH_HIP(hipGetLastError());
     });
 }
@@ -821,6 +858,7 @@ int wcsph_sim_sync(wcsph_sim* s) {
     return guarded([&] {
         WCSPH_REQUIRE(s, WCSPH_EINVAL, "null simulation");
         WCSPH_HIP(hipStreamSynchronize(s->st));
+        raise_if_failed(s);
     });
 }
 
@@ -839,6 +877,8 @@ int wcsph_sim_get_stats(wcsph_sim* s, wcsph_sim_stats* st) {
         st->escaped = h.escaped;
         st->max_count = (int32_t)h.maxcnt;
         st->overflow = (int32_t)h.overflow;
+        st->nonfinite = (int32_t)h.nonfinite;
+        st->overfull = (int32_t)h.overfull;
     });
 }
 
@@ -1085,6 +1125,7 @@ int wcsph_sim_save(wcsph_sim* s, const char* path) {
                       "a restart file holds the state after wcsph_sim_init or a step");
         WCSPH_HIP(hipSetDevice(s->device));
         WCSPH_HIP(hipStreamSynchronize(s->st));
+        raise_if_failed(s);  // a diverged state is not worth resuming
         HostState hs;
         fetch_state(s, hs);
         Scalars sc{};

@@ -52,9 +52,31 @@ struct Scalars {  // device-resident step scalars
     // and binning passes: kept 256 bytes away from the clock so that reads
     // of dt do not queue behind them in L2
     unsigned long long pairs_last, pairs_total;
-    unsigned int escaped, maxcnt, overflow, pad;
+    unsigned int escaped, maxcnt, overflow;
+    // sticky failure flags (cleared only by a full reset): particles whose
+    // position is not finite (binned into the dead bucket past the last
+    // cell), and cells holding more than kRankCap particles
+    unsigned int nonfinite, overfull, pad_flags[3];
 };
 static_assert(__builtin_offsetof(Scalars, pairs_last) == 256, "counters on their own cache lines");
+static_assert(__builtin_offsetof(Scalars, nonfinite) == 256 + 28, "failure flags are one 8-byte word");
+
+// A state with a non-finite position or an overfull cell has diverged: the
+// passes that scan cells then do no work (a diverged state could pile every
+// particle into one cell and make each scan quadratic in n), and the host
+// API reports the failure.
+__device__ __forceinline__ bool sim_failed(const Scalars* s) { return (s->nonfinite | s->overfull) != 0u; }
+
+// Particles a cell may hold for the ranked gather of k_reorder (beyond it
+// the cell is flagged overfull).
+constexpr uint32_t kRankCap = 256;
+
+// The dead bucket: the cell key past the grid's last cell, for particles
+// whose position is not finite. No stencil visits it.
+template <int D>
+__device__ __forceinline__ uint32_t dead_key(const Grid& g) {
+    return (uint32_t)g.nc[0] * (uint32_t)g.nc[1] * (uint32_t)(D == 3 ? g.nc[2] : 1);
+}
 
 // Per-block partial results (one slot per block of the particle kernels),
 // reduced in index order by k_reduce_partials: no same-address atomics.
@@ -94,8 +116,8 @@ __device__ __forceinline__ void cell_of(const Grid& g, const double* x, int* c, 
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         const double u = (x[d] - g.lo[d]) * g.invL;
-        int k = (int)floor(u);
-        if (!(u >= 0.0)) { k = 0; esc = true; }  // also catches NaN
+        int k = (int)floor(fmin(fmax(u, -1.0), 2.0e9));  // in int range before the clamps
+        if (!(u >= 0.0)) { k = 0; esc = true; }
         if (k > g.nc[d] - 1) { k = g.nc[d] - 1; esc = true; }
         c[d] = k;
     }
@@ -144,6 +166,8 @@ __global__ __launch_bounds__(kScalarsThreads) void k_scalars(Scalars* s, const d
         s->step = 0;
         s->pairs_total = 0;
         s->overflow = 0;
+        s->nonfinite = 0;
+        s->overfull = 0;
     }
     if (mode == 1) {
         s->dt = sh[0];
@@ -189,12 +213,20 @@ __global__ void k_bin(Master m, const double* __restrict__ acc, const double* __
                 }
             }
         }
-        int c[3] = {0, 0, 0};
-        bool esc = false;
-        cell_of<D>(g, x, c, esc);
-        k = cell_key<D>(g, c);
+        bool finite = true;
+#pragma unroll
+        for (int d = 0; d < D; ++d) finite = finite && isfinite(x[d]);
+        if (finite) {
+            int c[3] = {0, 0, 0};
+            bool esc = false;
+            cell_of<D>(g, x, c, esc);
+            k = cell_key<D>(g, c);
+            if (esc) atomicAdd(&s->escaped, 1u);
+        } else {
+            k = dead_key<D>(g);
+            atomicAdd(&s->nonfinite, 1u);
+        }
         key[i] = k;
-        if (esc) atomicAdd(&s->escaped, 1u);
     }
     const int lane = threadIdx.x & 63;
     const uint32_t kp = (uint32_t)__shfl_up((int)k, 1, 64);
@@ -352,6 +384,40 @@ __device__ __forceinline__ void key_cell(const Grid& g, uint32_t key, int* c) {
     c[2] = D == 3 ? (int)(key / ((uint32_t)g.nc[0] * (uint32_t)g.nc[1])) : 0;
 }
 
+// The member of slot range [s0, s1) that slot p takes in cell order: the
+// one with the (p - s0)-th smallest previous index (the scatter fills a
+// cell's slots in atomic order; this makes the order a function of the
+// state alone). Cells of up to 8 are ranked in registers; larger ones
+// (callers cap them at kRankCap) by a search quadratic in the count.
+__host__ __device__ inline uint32_t ranked_member(const uint32_t* perm, uint32_t s0, uint32_t s1, uint32_t p) {
+    const uint32_t cnt = s1 - s0, want = p - s0;
+    uint32_t o = perm[p];
+    constexpr uint32_t kC = 8;
+    if (cnt > 1 && cnt <= kC) {
+        uint32_t mem[kC];
+#pragma unroll
+        for (uint32_t t = 0; t < kC; ++t) mem[t] = t < cnt ? perm[s0 + t] : 0xffffffffu;
+#pragma unroll
+        for (uint32_t t = 0; t < kC; ++t) {
+            uint32_t r = 0;
+#pragma unroll
+            for (uint32_t u = 0; u < kC; ++u) r += mem[u] < mem[t] ? 1u : 0u;
+            if (t < cnt && r == want) o = mem[t];
+        }
+    } else if (cnt > kC) {
+        for (uint32_t q = s0; q < s1; ++q) {
+            const uint32_t cand = perm[q];
+            uint32_t r = 0;
+            for (uint32_t q2 = s0; q2 < s1; ++q2) r += perm[q2] < cand ? 1u : 0u;
+            if (r == want) {
+                o = cand;
+                break;
+            }
+        }
+    }
+    return o;
+}
+
 struct Packed {  // FP32 force records in cell order
     float4* px;
     float4* pv;
@@ -376,7 +442,7 @@ template <int D, class Extra>
 __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__ perm,
                                                     const uint32_t* __restrict__ key,
                                                     const uint2* __restrict__ crange, Master src, Master dst,
-                                                    const double* __restrict__ acc_old, const Scalars* s, int step,
+                                                    const double* __restrict__ acc_old, Scalars* s, int step,
                                                     int keep_vel, Packed pk, Grid g, double* __restrict__ pr64,
                                                     double* __restrict__ c64, EosConsts e, float v_scale,
                                                     Extra extra, int64_t n) {
@@ -386,33 +452,13 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
     int c[3] = {0, 0, 0};
     double rho = 0.0, m = 0.0, pr = 0.0, cs = 0.0;
     if (live) {
-        uint32_t o = perm[p];
         const uint2 cr = crange[p];
-        const uint32_t k = key[o];
-        const uint32_t cnt = cr.y - cr.x, want = (uint32_t)p - cr.x;
-        constexpr uint32_t kC = 8;  // members ranked in registers
-        if (cnt > 1 && cnt <= kC) {
-            uint32_t mem[kC];
-#pragma unroll
-            for (uint32_t t = 0; t < kC; ++t) mem[t] = t < cnt ? perm[cr.x + t] : 0xffffffffu;
-#pragma unroll
-            for (uint32_t t = 0; t < kC; ++t) {
-                uint32_t r = 0;
-#pragma unroll
-                for (uint32_t u = 0; u < kC; ++u) r += mem[u] < mem[t] ? 1u : 0u;
-                if (t < cnt && r == want) o = mem[t];
-            }
-        } else if (cnt > kC) {
-            for (uint32_t q = cr.x; q < cr.y; ++q) {
-                const uint32_t cand = perm[q];
-                uint32_t r = 0;
-                for (uint32_t q2 = cr.x; q2 < cr.y; ++q2) r += perm[q2] < cand ? 1u : 0u;
-                if (r == want) {
-                    o = cand;
-                    break;
-                }
-            }
-        }
+        const uint32_t k = key[perm[p]];
+        // the dead bucket and overfull cells keep the scatter's order (the
+        // state has diverged; the flag makes the host API report it)
+        const bool ranked = k != dead_key<D>(g) && cr.y - cr.x <= kRankCap;
+        if (k != dead_key<D>(g) && cr.y - cr.x > kRankCap && (uint32_t)p == cr.x) atomicOr(&s->overfull, 1u);
+        const uint32_t o = ranked ? ranked_member(perm, cr.x, cr.y, (uint32_t)p) : perm[p];
         key_cell<D>(g, k, c);
         const bool moving = step && src.type[o] == 0;
         const double hdt = step ? 0.5 * s->dt : 0.0;
@@ -671,10 +717,11 @@ __device__ __forceinline__ void block_count_atomic(uint32_t cnt, Scalars* s, boo
 template <int D, int R>
 __global__ __launch_bounds__(kBlock) void k_neigh(Packed pk, const uint32_t* __restrict__ start, Grid g,
                                                   int64_t n, int K, float hmax, uint32_t* __restrict__ nbr,
-                                                  uint32_t* __restrict__ cnt, Partials P) {
+                                                  uint32_t* __restrict__ cnt, Partials P, const Scalars* s) {
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     uint32_t c_out = 0;
-    if (i < n) {
+    if (i < n && sim_failed(s)) cnt[i] = 0;
+    if (i < n && !sim_failed(s)) {
         const float4 xi4 = pk.px[i];
         const float xi[3] = {xi4.x, xi4.y, xi4.z};
         const float hi = pk.pq[i].w;
@@ -815,7 +862,8 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
     const int lane = threadIdx.x & 63;
     uint32_t* q = &qbuf[threadIdx.x >> 6][lane * (Q + 1)];
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-    const bool live = i < a.n;
+    const bool failed = sim_failed(a.s);
+    const bool live = i < a.n && !failed;
     const float4 xi4 = live ? a.pk.px[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 vi4 = live ? a.pk.pv[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 qi4 = live ? a.pk.pq[i] : make_float4(1.f, 0.f, 1.f, 1.f);
@@ -950,7 +998,7 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
         }
     }
     flush();
-    double cand = 1.0e30;
+    double cand = failed ? 0.0 : 1.0e30;  // a diverged state stops the clock
     if (live) {
         const int64_t n = a.n;
         double ad[D];
@@ -974,9 +1022,14 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
 // committed by k_apply_density once every particle has read the old values.
 template <int D, int R>
 __global__ __launch_bounds__(kBlock) void k_shepard(Master m, Packed pk, const uint32_t* __restrict__ start, Grid g,
-                                                    int64_t n, double cnorm, double* __restrict__ out) {
+                                                    int64_t n, double cnorm, double* __restrict__ out,
+                                                    const Scalars* sc) {
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (sim_failed(sc)) {  // a diverged state keeps its densities
+        out[i] = m.dens[i];
+        return;
+    }
     double xi[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) xi[d] = m.pos[d * n + i];
@@ -1106,8 +1159,8 @@ __global__ __launch_bounds__(kBlock) void k_final(FinalArgs f, double* __restric
                                                   const double* __restrict__ dti, const Scalars* s, Partials P,
                                                   int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double cand = 1.0e30;
-    if (i < n) {
+    double cand = sim_failed(s) ? 0.0 : 1.0e30;  // a diverged state stops the clock
+    if (i < n && cand > 0.0) {
         double a[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) a[d] = acc[d * n + i];
