@@ -1,0 +1,140 @@
+"""Full-size parity on the 3D dam break at dx = 0.0126 m (1,065,378
+particles), developed for 200 steps, where the column is deepest in
+smoothing lengths (H/h ~ 122) and the cancellation in the pressure gradient
+is worst:
+
+* the device-resident FP32 pass (k_nforce) against the FP64 oracle's force
+  pass over the whole state: acc and div v within 1e-5 normwise, dt within
+  1e-6, and the pair count equal to the oracle's count of the same
+  cell-local predicate;
+* the FDPS drop-in (dispatch / retrieve, k_walk_force) on the oracle's
+  FDPS-style walks of the same state, groups of at most 64 (16,7xx walks),
+  called in FDPS's order with n_walk_limit = 16,384 (src/main.cpp:112,
+  src/kernel.cuh:4), so the batch, chunk and staging logic runs at this
+  size: the same tolerances against the oracle over the same walks;
+* the mixed-precision pass at its stated bound (acc <= 1e-2, div v
+  <= 5e-3), with its error at this depth printed.
+
+Each test prints its oracle and device times.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+@pytest.fixture(scope="module")
+def c1(O, W):
+    t0 = time.perf_counter()
+    p = W.dam_break_params(3)
+    st = W.dam_break_state(W.dam_break_desc(3, 0.0126), p)
+    assert len(st["dens"]) == 1065378
+    with W.Simulation.from_state(p, st, precision="fp32") as sim:
+        sim.init()
+        sim.step(200)
+        d = sim.download()
+    state = dict(pos=d["pos"], vel=d["vel"], dens=d["dens"], mass=st["mass"], h=st["h"], type=st["type"])
+    op = O.sim_params(3, alpha=p.alpha, c_cfl=p.c_cfl, c0=p.c0, g=p.g, nu=p.nu)
+    s = O.State(3, len(state["dens"]))
+    s.x[:], s.v[:], s.dens[:], s.mass[:], s.h[:], s.type[:] = (state[k] for k in
+                                                               ("pos", "vel", "dens", "mass", "h", "type"))
+    s.eos(op)
+    t1 = time.perf_counter()
+    s.force(op, THREADS)
+    t2 = time.perf_counter()
+    print(f"C1 state after 200 steps: {t1 - t0:.1f} s; oracle force pass {t2 - t1:.2f} s on {THREADS} threads",
+          flush=True)
+    return p, op, state, s
+
+
+def test_c1_fp32_force_pass_matches_oracle(O, W, c1):
+    p, op, state, s = c1
+    t0 = time.perf_counter()
+    with W.Simulation.from_state(p, state, precision="fp32") as sim:
+        sim.force()
+        out = sim.download()
+        st = sim.stats()
+        lo, L, nc = sim.grid()
+        ms = sim.last_force_ms()["force"]
+    t1 = time.perf_counter()
+    counts = O.neighbour_counts_cells(s.x, s.h.astype(np.float32), lo, L, nc, THREADS)
+    t2 = time.perf_counter()
+    ea, ed = rel(out["acc"], s.a), rel(out["div_v"], s.divv)
+    print(f"fp32 at C1: acc {ea:.2e}, div_v {ed:.2e}, pairs {st['pairs_last']} (oracle {int(counts.sum())}); "
+          f"force kernel {ms:.3f} ms, device call {t1 - t0:.1f} s, oracle count {t2 - t1:.1f} s", flush=True)
+    assert st["escaped"] == 0 and st["overflow"] == 0
+    assert ea <= 1e-5 and ed <= 1e-5
+    assert np.allclose(out["dt"], s.dt_i, rtol=1e-6, atol=0)
+    assert st["pairs_last"] == int(counts.sum())
+
+
+def test_c1_fdps_dropin_matches_oracle(O, W, c1):
+    p, op, state, s = c1
+    n = s.n
+    epi_all = np.zeros(n, W.epi_dtype(3))
+    epj_all = np.zeros(n, W.epj_dtype(3))
+    for a in (epi_all, epj_all):
+        a["pos"], a["vel"], a["smth"], a["dens"], a["pres"], a["snds"], a["visc"] = \
+            s.x.T, s.v.T, s.h, s.dens, s.pres, s.snds, p.nu
+        a["tag"] = np.arange(n)
+    epj_all["mass"] = s.mass
+    epj_all["type"] = s.type
+    t0 = time.perf_counter()
+    go, gi, no, ni = O.make_walks(s.x, r_search=2 * float(s.h.max()), group_max=64)
+    epi = [epi_all[gi[go[w]:go[w + 1]]] for w in range(len(go) - 1)]
+    epj = [epj_all[ni[no[w]:no[w + 1]]] for w in range(len(no) - 1)]
+    nw = len(epi)
+    assert nw > 16384  # two batches at FDPS's walk limit
+    t1 = time.perf_counter()
+    W.init(p)
+    limit = 16384
+    res = [np.zeros(len(e), W.res_dtype(3)) for e in epi]
+    prev = None
+    for b in range(0, nw, limit):  # FDPS's order: retrieve(k - 1), then dispatch(k)
+        e = slice(b, min(nw, b + limit))
+        if prev is not None:
+            assert W.HydrRetrieveKernel(0, prev.stop - prev.start, [len(x) for x in epi[prev]], res[prev]) == 0, \
+                W.last_error()
+        assert W.HydrDispatchKernel(0, e.stop - e.start, epi[e], [len(x) for x in epi[e]], epj[e],
+                                    [len(x) for x in epj[e]]) == 0, W.last_error()
+        prev = e
+    assert W.HydrRetrieveKernel(0, prev.stop - prev.start, [len(x) for x in epi[prev]], res[prev]) == 0
+    W.finalize()
+    t2 = time.perf_counter()
+    got = np.concatenate(res)
+    fp = O.force_params(3, alpha=p.alpha, beta=p.beta, eps=p.eps, c_cfl=p.c_cfl, vsig_mode=p.vsig_mode)
+    io = np.concatenate([[0], np.cumsum([len(e) for e in epi])])
+    jo = np.concatenate([[0], np.cumsum([len(e) for e in epj])])
+    ref = O.hydro_force_walks(fp, np.concatenate(epi), io, np.concatenate(epj), jo, nthreads=THREADS)
+    t3 = time.perf_counter()
+    ea, ed = rel(got["acc"], ref["acc"]), rel(got["div_v"], ref["div_v"])
+    print(f"drop-in at C1: {nw} walks, {int(jo[-1])} j records; acc {ea:.2e}, div_v {ed:.2e}; walks {t1 - t0:.1f} s, "
+          f"dispatch+retrieve {t2 - t1:.2f} s, oracle {t3 - t2:.1f} s", flush=True)
+    assert ea <= 1e-5 and ed <= 1e-5
+    assert np.allclose(got["dt"], ref["dt"], rtol=1e-6, atol=0)
+    # and the walks' results are the whole-state pass's, particle by particle
+    order = gi
+    assert rel(got["acc"], s.a.T[order]) <= 1e-5
+
+
+def test_c1_mixed_force_pass_within_bound(O, W, c1):
+    p, op, state, s = c1
+    with W.Simulation.from_state(p, state, precision="mixed") as sim:
+        sim.force()
+        out = sim.download()
+        st = sim.stats()
+        ms = sim.last_force_ms()["force"]
+    ea, ed = rel(out["acc"], s.a), rel(out["div_v"], s.divv)
+    edt = float(np.max(np.abs(out["dt"] / s.dt_i - 1)))
+    print(f"mixed at C1 (H/h ~ 122): acc {ea:.2e}, div_v {ed:.2e}, dt {edt:.2e}; pairs {st['pairs_last']}; "
+          f"force kernel {ms:.3f} ms", flush=True)
+    assert ea <= 1e-2 and ed <= 5e-3 and edt <= 5e-3

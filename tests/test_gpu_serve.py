@@ -92,3 +92,31 @@ def test_serving_rejects_oversized_request(W):
     reqs = small_stream(W)[:1]
     with pytest.raises(W.WcsphError, match="budget"):
         W.serve(reqs, hbm_budget_bytes=1 << 20)
+
+
+def test_serving_c4_stream_results_equal_alone(W):
+    # a stream of the serving configuration's shape: seeded Poisson arrivals
+    # (4/s), 2D and 3D dam breaks of 20k-2M particles (log-uniform), column
+    # aspect 1-4, alpha 0.02-0.2, 500-5000 steps, plus one 3D request of 2M
+    # particles; every request's result in the mix equals its result served
+    # alone, bit for bit, and the completion latencies are reported
+    import time
+    reqs = W.poisson_workload(2026, 6, 4.0, precision="mixed")
+    big = W.make_request(3, 2_000_000, 2.0, 0.1, 600, reqs[2].arrival_s, "mixed")
+    reqs.insert(3, big)
+    t0 = time.perf_counter()
+    mix = W.serve(reqs, max_slots=4, chunk_steps=50)
+    wall = time.perf_counter() - t0
+    assert all(r["status"] == 0 for r in mix), [r["status"] for r in mix]
+    assert max(r["n"] for r in mix) >= 1_900_000 and min(r["steps"] for r in mix) >= 500
+    lat = np.array([r["latency_s"] for r in mix])
+    pairs = sum(r["pairs"] for r in mix)
+    makespan = max(r["end_s"] for r in mix)
+    print(f"C4 stream: {len(reqs)} requests, n {[r['n'] for r in mix]}, steps {[r['steps'] for r in mix]}; "
+          f"p50 {np.percentile(lat, 50):.3f} s, p99 {np.percentile(lat, 99):.3f} s, makespan {makespan:.2f} s, "
+          f"{pairs / makespan:.3e} pairs/s (wall {wall:.1f} s)", flush=True)
+    for k, r in enumerate(reqs):
+        r.arrival_s = 0.0
+        alone = W.serve([r], max_slots=1, chunk_steps=50)[0]
+        for key in KEYS:
+            assert alone[key] == mix[k][key], (k, key)

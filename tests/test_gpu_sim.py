@@ -386,3 +386,123 @@ def test_force_variants_beta_vsig_cells(O, W, dt_neighbours, dim, precision, tol
     assert rel(m["acc"], s.a) <= tol_acc and rel(m["div_v"], s.divv) <= tol_div
     rtol = {"mixed": 5e-3, "fp32": 1e-6, "fp64": 1e-12}[precision]
     assert np.allclose(m["dt"], dt_neighbours(s, op, grid), rtol=rtol)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp64", 1e-12), ("mixed", 5e-2)])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_laminar_viscosity_shear_device(O, W, dim, precision, tol):
+    # the Morris laminar term (src/force.h:53) as the only force: a lattice
+    # block of fluid at rho0 (P = 0), alpha = 0, nu = 1e-2 m^2/s and the
+    # shear flow v_x = A y^2. The device pass matches the oracle over the
+    # whole block, and in the interior (2h from every face) both give
+    # a_x = nu lap v_x = 2 nu A to the lattice's 3% (the oracle's own closed
+    # form, test_laminar_viscosity_shear_closed_form). The mixed path holds
+    # v / c0 in FP16 (2^-11 relative): at the block's edge in 2D neighbour
+    # differences are 6% of the velocity itself, so each loses ~1% and the
+    # viscous-only force ~2% (2.2e-2 measured in 2D); its bound
+    # here is 5e-2, which still fails a coefficient off by a factor of two.
+    dx, A, nu = 0.02, 2.5, 1.0e-2
+    m = 24 if dim == 3 else 64
+    g = np.stack(np.meshgrid(*[np.arange(m)] * dim, indexing="ij"), 0).reshape(dim, -1) * dx
+    g -= g.mean(axis=1, keepdims=True)
+    n = g.shape[1]
+    p = W.dam_break_params(dim, alpha=0.0, nu=nu)
+    op = O.sim_params(dim, alpha=0.0, c_cfl=p.c_cfl, c0=p.c0, g=p.g, nu=nu)
+    s = O.State(dim, n)
+    s.x[:] = g
+    s.v[0] = A * g[1] ** 2
+    s.dens[:], s.mass[:], s.h[:] = 1000.0, 1000.0 * dx ** dim, 1.3 * dx
+    s.eos(op)
+    s.force(op, 8)
+    with W.Simulation.from_state(p, state_dict(s), precision=precision) as sim:
+        sim.force()
+        out = sim.download()
+    inner = np.all(np.abs(g) < g.max() - 2.6 * dx, axis=0)
+    print(f"{dim}D {precision}: rel err vs oracle {rel(out['acc'], s.a):.2e}, interior a_x / (2 nu A) in "
+          f"[{out['acc'][0][inner].min() / (2 * nu * A):.4f}, {out['acc'][0][inner].max() / (2 * nu * A):.4f}]")
+    assert rel(out["acc"], s.a) <= tol
+    assert inner.sum() > 100
+    assert np.allclose(s.a[0][inner], 2 * nu * A, rtol=0.03)
+    if precision != "mixed":
+        # the FP16 velocity rounding is the same in absolute terms over the
+        # block, so where a_x is 20x smaller than at the faces it is not
+        # resolved on the mixed path
+        assert np.allclose(out["acc"][0][inner], 2 * nu * A, rtol=0.03 + tol)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "mixed"])
+def test_viscous_time_step_bound_device(W, precision):
+    # getTimeStepGlobal's explicit-diffusion bound 0.125 h^2 / nu
+    # (src/integral.h:8) binding: one isolated particle, h = 0.01 m,
+    # nu = 1 m^2/s, so dt = 1.25e-5 s (C h / c = 4.0e-5 s, 0.25 sqrt(h/g) =
+    # 8.0e-3 s); the bound is formed in FP64 on every path
+    p = W.dam_break_params(3, nu=1.0)
+    st = dict(pos=np.array([[0.3], [0.2], [5.0]]), vel=np.zeros((3, 1)), dens=np.array([1000.0]),
+              mass=np.array([1e-3]), h=np.array([0.01]), type=np.array([0], np.int32))
+    with W.Simulation.from_state(p, st, precision=precision) as sim:
+        sim.init()
+        sim.step(10)
+        t = sim.stats()["time"]
+    assert t == pytest.approx(10 * 1.25e-5, rel=1e-13)
+
+
+def _with_extra(st, pos, vel):
+    # the 2D dam break plus extra fluid particles at pos / vel ([2][k])
+    k = pos.shape[1]
+    out = dict(pos=np.concatenate([st["pos"], pos], axis=1), vel=np.concatenate([st["vel"], vel], axis=1))
+    for key, v in (("dens", 1000.0), ("mass", st["mass"][0]), ("h", st["h"][0]), ("type", 0)):
+        out[key] = np.concatenate([st[key], np.full(k, v, st[key].dtype)])
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed", "fp64"])
+def test_nonfinite_position_reports_divergence(W, precision):
+    # a fluid particle near the largest double moving at 1.79e308 m/s
+    # overflows to +inf in the first drift: the binning pass puts it into
+    # the dead bucket past the last cell and counts it, the passes that scan
+    # cells then do no work and stop the clock, and the host API reports
+    # WCSPH_EDIVERGED (sync at once, step refuses to queue more)
+    p = W.dam_break_params(2)
+    st = _with_extra(W.dam_break_state(W.dam_break_desc(2, 0.05), p), np.full((2, 1), 1.79769e308),
+                     np.full((2, 1), 1.79e308))
+    with W.Simulation.from_state(p, st, precision=precision, box_lo=[-0.2, -0.2], box_hi=[4.2, 4.5]) as sim:
+        sim.init()
+        sim.sync()  # finite at init: clamped into the corner cell
+        assert sim.stats()["escaped"] == 1
+        sim.step(4)
+        with pytest.raises(W.WcsphError) as e:
+            sim.sync()
+        assert e.value.code == W._abi.WCSPH_EDIVERGED and "non-finite" in str(e.value)
+        s1 = sim.stats()
+        assert s1["nonfinite"] >= 1 and s1["overfull"] == 0
+        with pytest.raises(W.WcsphError) as e:
+            sim.step(2)
+        assert e.value.code == W._abi.WCSPH_EDIVERGED
+        assert sim.stats()["time"] == s1["time"] and sim.stats()["step"] == s1["step"]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_overfull_cell_reports_divergence(W, precision):
+    # 300 particles on one point fill one cell beyond the 256 the ranked
+    # gather orders: the cell is flagged overfull at the first binning, the
+    # force pass does no work and sync reports WCSPH_EDIVERGED
+    p = W.dam_break_params(2)
+    st = _with_extra(W.dam_break_state(W.dam_break_desc(2, 0.05), p), np.tile([[2.0], [1.5]], (1, 300)),
+                     np.zeros((2, 300)))
+    with W.Simulation.from_state(p, st, precision=precision) as sim:
+        sim.init()
+        with pytest.raises(W.WcsphError) as e:
+            sim.sync()
+        assert e.value.code == W._abi.WCSPH_EDIVERGED and "more than 256" in str(e.value)
+        assert sim.stats()["overfull"] == 1
+
+
+def test_upload_rejects_nonfinite(W):
+    p = W.dam_break_params(2)
+    st = W.dam_break_state(W.dam_break_desc(2, 0.05), p)
+    for key in ("pos", "vel", "dens"):
+        bad = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in st.items()}
+        bad[key].reshape(-1)[5] = np.nan
+        with pytest.raises(W.WcsphError) as e:
+            W.Simulation.from_state(p, bad)
+        assert e.value.code == W._abi.WCSPH_EINVAL

@@ -241,6 +241,26 @@ class Simulation:
         keys = ["mass", "mom_x", "mom_y", "mom_z", "kinetic", "potential", "internal", "front", "column", "runup"]
         return dict(zip(keys, out))
 
+    def sample_marks(self, marks, zfront, xcol, xwall=1.0e300, chunk=50, on_sample=None):
+        """Step until the clock passes each time in `marks` (ascending),
+        taking single steps into each mark so the sample falls within one dt
+        past it, and return one diagnostics dict per mark (with its "time"
+        and "step"). Between marks steps are queued `chunk` at a time;
+        on_sample(stats, diagnostics), if given, sees every chunk."""
+        out = []
+        todo = list(marks)
+        while todo:
+            s = self.stats()
+            self.step(1 if s["time"] + (chunk + 1) * s["dt"] >= todo[0] else chunk)
+            s = self.stats()
+            d = self.diagnostics(zfront, xcol, xwall)
+            if on_sample is not None:
+                on_sample(s, d)
+            if s["time"] >= todo[0]:
+                out.append(dict(d, time=s["time"], step=s["step"]))
+                todo.pop(0)
+        return out
+
     def set_graphs(self, enable=True):
         check(lib().wcsph_sim_set_graphs(self._h, 1 if enable else 0))
 
