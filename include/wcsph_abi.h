@@ -334,6 +334,37 @@ int wcsph_sim_set_clock(wcsph_sim* s, double time, int64_t step);
  * counterpart in the reference. */
 int wcsph_sim_params(wcsph_sim* s, wcsph_params* p, wcsph_sim_config* cfg, int64_t* n);
 
+/* ---------------- 5. one force pass on device-resident particles ---------------- */
+/* The particles of a pass already live in device memory, owned by the
+ * caller, in the caller's order. */
+typedef struct {
+    int64_t n;              /* particles */
+    double box_lo[3];       /* cell-grid domain: particles outside it are clamped into */
+    double box_hi[3];       /*   the edge cells (their neighbours are still all found) */
+    double h_max;           /* the largest smoothing length in h[]: cells are h_max
+                               wide (two per support 2 h_max); a larger h would
+                               miss neighbours */
+} wcsph_pass_desc;
+
+/* One hydro force pass of CalcHydroForce (src/force.h:31-60) over all n
+ * particles, as calcForceAllAndWriteBack runs it once per step
+ * (src/main.cpp:75,118), with FDPS's tree and the dispatch/retrieve copies
+ * replaced by a cell list built on the device: pressure and sound speed from
+ * the Tait law of dens (CalcPressure, src/force.h:3-9), the general pair body
+ * (per-pair mean smoothing length, so h may vary), the neighbour set
+ * 0 < r^2 < (h_i + h_j)^2 of the device-resident path. Inputs pos/vel
+ * [dim][n], dens, mass, h [n] and outputs acc [dim][n] (hydro only), div_v,
+ * dt_i [n] are device pointers (FP64) in the caller's order; pairs, if not
+ * NULL, is a device int64 that receives the pair count, or -1 when a
+ * position was not finite or a cell held more than 256 particles (then the
+ * outputs are not written). Queued on `stream` (a hipStream_t; NULL for
+ * the default stream); returns without waiting. Workspace (about
+ * 76 bytes per particle and 8 per cell) is owned by the library, one per
+ * device, grown on demand; calls that overlap on one device race on it. */
+int wcsph_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const double* pos, const double* vel,
+                     const double* dens, const double* mass, const double* h, double* acc, double* div_v,
+                     double* dt_i, int64_t* pairs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

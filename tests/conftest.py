@@ -50,3 +50,59 @@ def dt_neighbours(O):
         np.maximum.at(vsig, I, -mu if op.vsig_mode else mu)
         return op.c_cfl * s.h / (s.snds + vsig)
     return dt
+
+
+class DeviceMemory:
+    """Device buffers for the entry points that take device pointers,
+    through the HIP runtime libwcsph.so itself links (libamdhip64.so.7),
+    so the pointers belong to the library's runtime."""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        self.hip = C.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.hip.hipFree.argtypes = [C.c_void_p]
+        self.hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.hip.hipDeviceSynchronize.argtypes = []
+        self.live = []
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    def alloc(self, nbytes):
+        p = self.C.c_void_p()
+        self._ok(self.hip.hipMalloc(self.C.byref(p), max(int(nbytes), 1)), "hipMalloc")
+        self.live.append(p.value)
+        return p.value
+
+    def upload(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        p = self.alloc(a.nbytes)
+        self._ok(self.hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
+        return p
+
+    def download(self, p, shape, dtype):
+        import numpy as np
+        out = np.empty(shape, dtype)
+        self.sync()
+        self._ok(self.hip.hipMemcpy(out.ctypes.data, p, out.nbytes, 2), "hipMemcpy D2H")
+        return out
+
+    def sync(self):
+        self._ok(self.hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def free_all(self):
+        self.sync()
+        for p in self.live:
+            self.hip.hipFree(p)
+        self.live = []
+
+
+@pytest.fixture
+def devmem(W):
+    m = DeviceMemory()
+    yield m
+    m.free_all()

@@ -66,6 +66,11 @@ class RequestResult(C.Structure):
                 ("precision_used", C.c_int32)]
 
 
+class PassDesc(C.Structure):
+    """wcsph_pass_desc."""
+    _fields_ = [("n", C.c_int64), ("box_lo", C.c_double * 3), ("box_hi", C.c_double * 3), ("h_max", C.c_double)]
+
+
 class ServeConfig(C.Structure):
     """wcsph_serve_config."""
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_slots", C.c_int32), ("chunk_steps", C.c_int32)]
@@ -99,7 +104,7 @@ EXPORTS = [
     "wcsph_sim_diagnostics", "wcsph_sim_last_force_ms", "wcsph_sim_set_timing", "wcsph_sim_timing",
     "wcsph_sim_grid", "wcsph_sim_set_graphs", "wcsph_sim_query", "wcsph_serve",
     "wcsph_snapshot_write_ascii", "wcsph_snapshot_read_ascii", "wcsph_sim_write_ascii", "wcsph_sim_save",
-    "wcsph_sim_restore", "wcsph_sim_set_clock", "wcsph_sim_params",
+    "wcsph_sim_restore", "wcsph_sim_set_clock", "wcsph_sim_params", "wcsph_force_pass",
 ]
 
 _lib = None
@@ -159,6 +164,7 @@ def lib():
         "wcsph_sim_restore": (C.c_int, [C.c_char_p, I32, C.POINTER(VP)]),
         "wcsph_sim_set_clock": (C.c_int, [VP, F64, I64]),
         "wcsph_sim_params": (C.c_int, [VP, C.POINTER(Params), C.POINTER(SimConfig), C.POINTER(I64)]),
+        "wcsph_force_pass": (C.c_int, [C.POINTER(Params), C.POINTER(PassDesc)] + [VP] * 10),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1,0 +1,41 @@
+// One stateless hydro force pass over caller-owned device arrays: the host
+// side shared by wcsph_force_pass (library-owned workspace, grown with
+// hipMalloc) and its variants that bring their own workspace memory.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+#include "common.hpp"
+#include "sim_kernels.hpp"
+
+namespace wcsph {
+
+// Byte offsets of the pass's workspace arrays (256-byte aligned).
+struct PassLayout {
+    size_t key, slot, perm, id, crange, px, pv, pq, pcell, count, start, part, sc, total;
+};
+
+// The cell grid of a pass and its size, from the caller's description
+// (throws Error on a bad description).
+struct PassGrid {
+    Grid g;
+    int64_t ncell;
+};
+PassGrid pass_grid(const wcsph_params* p, const wcsph_pass_desc* d);
+
+PassLayout pass_layout(int64_t n, int64_t ncell);
+
+// Queue the pass on `st` with the workspace at `ws` (at least
+// pass_layout(n, ncell).total bytes, 256-byte aligned, device memory).
+// Every array the pass reads before writing is cleared by a kernel first,
+// so the workspace may hold anything on entry.
+void run_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const PassGrid& pg, void* ws,
+                    const double* pos, const double* vel, const double* dens, const double* mass, const double* h,
+                    double* acc, double* div_v, double* dt_i, int64_t* pairs, hipStream_t st);
+
+// Argument checks shared by the entry points.
+void check_pass_args(const wcsph_params* p, const wcsph_pass_desc* d, const double* pos, const double* vel,
+                     const double* dens, const double* mass, const double* h, double* acc, double* div_v,
+                     double* dt_i);
+
+}  // namespace wcsph

@@ -347,6 +347,7 @@ static void launch_nforce(wcsph_sim* s) {
     a.s = s->sc;
     a.fused_final = s->fuse_final ? 1 : 0;
     a.fin = final_args(s, 1);
+    a.out_index = nullptr;
     hipLaunchKernelGGL((k_nforce<D, R, Q, UH, VS>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
 }
 

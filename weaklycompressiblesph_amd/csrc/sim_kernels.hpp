@@ -132,7 +132,7 @@ __device__ __forceinline__ uint32_t cell_key(const Grid& g, const int* c) {
 // per-block bounds, time += dt, step += 1; 2: reset the per-pass counters
 // only. One block of kScalarsThreads threads, four loads in flight each.
 constexpr int kScalarsThreads = 1024;
-__global__ __launch_bounds__(kScalarsThreads) void k_scalars(Scalars* s, const double* __restrict__ dtmin,
+static __global__ __launch_bounds__(kScalarsThreads) void k_scalars(Scalars* s, const double* __restrict__ dtmin,
                                                               int64_t nblk, int mode) {
     __shared__ double sh[kScalarsThreads / 64];
     if (mode == 1) {
@@ -276,7 +276,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return before + incl - v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t m,
+static __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t m,
                                                          uint32_t* __restrict__ part) {
     __shared__ uint32_t sh[kBlock / 64];
     const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restri
 }
 
 constexpr int kScanTilesThreads = 1024;
-__global__ __launch_bounds__(kScanTilesThreads) void k_scan_tiles(uint32_t* __restrict__ part, int64_t ntiles) {
+static __global__ __launch_bounds__(kScanTilesThreads) void k_scan_tiles(uint32_t* __restrict__ part, int64_t ntiles) {
     __shared__ uint32_t sh[kScanTilesThreads / 64];
     uint32_t carry = 0;
     for (int64_t b = 0; b < ntiles; b += 4 * kScanTilesThreads) {
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kScanTilesThreads) void k_scan_tiles(uint32_t* __re
 
 // Last scan phase: each tile scans its own counts from its offset, writes
 // start and clears count for the next binning pass.
-__global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ count, int64_t m,
+static __global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ count, int64_t m,
                                                         const uint32_t* __restrict__ part,
                                                         uint32_t* __restrict__ start) {
     __shared__ uint32_t sh[kBlock / 64];
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ co
 
 // Counting sort, pass 2: scatter old indices to their cell slots, each
 // with its cell's slot range [s0, s1).
-__global__ void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
+static __global__ void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
                           const uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
                           uint2* __restrict__ crange, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -661,7 +661,7 @@ __device__ __forceinline__ void reduce_partials_block(const Partials& P, Scalars
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_reduce_partials(Partials P, Scalars* s, int what) {
+static __global__ __launch_bounds__(kBlock) void k_reduce_partials(Partials P, Scalars* s, int what) {
     __shared__ ReduceScratch sh;
     reduce_partials_block(P, s, what, sh);
 }
@@ -826,6 +826,8 @@ struct NFArgs {
     Scalars* s;
     int fused_final;  // run the final stage (FinalArgs) in the epilogue
     FinalArgs fin;
+    // outputs of cell-order particle i go to index out_index[i] (null: i)
+    const uint32_t* out_index;
 };
 
 
@@ -1005,11 +1007,12 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
 #pragma unroll
         for (int d = 0; d < D; ++d) ad[d] = (double)acc[d];
         const double dti = a.c_cfl * (double)I.h / ((double)I.c + (double)vsig);
-        a.divv[i] = (double)divv / (double)I.rho;
-        a.dti[i] = dti;
+        const int64_t o = a.out_index ? (int64_t)a.out_index[i] : i;
+        a.divv[o] = (double)divv / (double)I.rho;
+        a.dti[o] = dti;
         if (a.fused_final) cand = final_particle<D>(a.fin, a.s, i, n, ad, dti);
 #pragma unroll
-        for (int d = 0; d < D; ++d) a.acc[d * n + i] = ad[d];
+        for (int d = 0; d < D; ++d) a.acc[d * n + o] = ad[d];
     }
     block_count_atomic(total, a.s, true, a.fused_final ? &a.P.dtmin[blockIdx.x] : nullptr, cand);
 }
