@@ -128,32 +128,6 @@ PassGrid pass_grid(const wcsph_params* p, const wcsph_pass_desc* d) {
     return pg;
 }
 
-PassLayout pass_layout(int64_t n, int64_t ncell) {
-    PassLayout L{};
-    size_t off = 0;
-    auto take = [&](size_t bytes) {
-        const size_t at = off;
-        off += (bytes + 255) & ~(size_t)255;
-        return at;
-    };
-    L.key = take(4 * (size_t)n);
-    L.slot = take(4 * (size_t)n);
-    L.perm = take(4 * (size_t)n);
-    L.id = take(4 * (size_t)n);
-    L.crange = take(8 * (size_t)n);
-    L.px = take(16 * (size_t)n);
-    L.pv = take(16 * (size_t)n);
-    L.pq = take(16 * (size_t)n);
-    L.pcell = take(4 * (size_t)n);
-    // one bucket past the last cell: the dead bucket of non-finite positions
-    L.count = take(4 * (size_t)(ncell + 1));
-    L.start = take(4 * (size_t)(ncell + 2));
-    L.part = take(4 * (size_t)ceil_div(ncell + 1, kScanTile));
-    L.sc = take(sizeof(Scalars));
-    L.total = off;
-    return L;
-}
-
 void run_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const PassGrid& pg, void* ws,
                     const double* pos, const double* vel, const double* dens, const double* mass, const double* h,
                     double* acc, double* div_v, double* dt_i, int64_t* pairs, hipStream_t st) {

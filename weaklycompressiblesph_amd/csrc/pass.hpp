@@ -23,7 +23,31 @@ struct PassGrid {
 };
 PassGrid pass_grid(const wcsph_params* p, const wcsph_pass_desc* d);
 
-PassLayout pass_layout(int64_t n, int64_t ncell);
+constexpr PassLayout pass_layout(int64_t n, int64_t ncell) {
+    PassLayout L{};
+    size_t off = 0;
+    auto take = [&off](size_t bytes) {
+        const size_t at = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    L.key = take(4 * (size_t)n);
+    L.slot = take(4 * (size_t)n);
+    L.perm = take(4 * (size_t)n);
+    L.id = take(4 * (size_t)n);
+    L.crange = take(8 * (size_t)n);
+    L.px = take(16 * (size_t)n);
+    L.pv = take(16 * (size_t)n);
+    L.pq = take(16 * (size_t)n);
+    L.pcell = take(4 * (size_t)n);
+    // one bucket past the last cell: the dead bucket of non-finite positions
+    L.count = take(4 * (size_t)(ncell + 1));
+    L.start = take(4 * (size_t)(ncell + 2));
+    L.part = take(4 * (size_t)((ncell + 1 + kScanTile - 1) / kScanTile));
+    L.sc = take(sizeof(Scalars));
+    L.total = off;
+    return L;
+}
 
 // Queue the pass on `st` with the workspace at `ws` (at least
 // pass_layout(n, ncell).total bytes, 256-byte aligned, device memory).
