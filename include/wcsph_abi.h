@@ -365,6 +365,20 @@ int wcsph_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const doub
                      const double* dens, const double* mass, const double* h, double* acc, double* div_v,
                      double* dt_i, int64_t* pairs, void* stream);
 
+/* wcsph_force_pass without any allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, kernels and results as wcsph_force_pass, but the workspace is a
+ * fixed device array of the library (about 56 MB, one copy per GPU, held
+ * until the process ends), so it takes at most 524,288 particles and a grid
+ * of at most 2,097,152 cells of h_max (WCSPH_EOVERFLOW beyond, before any
+ * work is queued). Calls no hipMalloc, hipFree or other allocating routine.
+ * All calls on one GPU share that GPU's array: calls that overlap on one GPU
+ * race. */
+int wcsph_force_pass_noalloc(const wcsph_params* p, const wcsph_pass_desc* d, const double* pos,
+                             const double* vel, const double* dens, const double* mass, const double* h,
+                             double* acc, double* div_v, double* dt_i, int64_t* pairs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,7 @@ EXPORTS = [
     "wcsph_sim_grid", "wcsph_sim_set_graphs", "wcsph_sim_query", "wcsph_serve",
     "wcsph_snapshot_write_ascii", "wcsph_snapshot_read_ascii", "wcsph_sim_write_ascii", "wcsph_sim_save",
     "wcsph_sim_restore", "wcsph_sim_set_clock", "wcsph_sim_params", "wcsph_force_pass",
+    "wcsph_force_pass_noalloc",
 ]
 
 _lib = None
@@ -165,6 +166,7 @@ def lib():
         "wcsph_sim_set_clock": (C.c_int, [VP, F64, I64]),
         "wcsph_sim_params": (C.c_int, [VP, C.POINTER(Params), C.POINTER(SimConfig), C.POINTER(I64)]),
         "wcsph_force_pass": (C.c_int, [C.POINTER(Params), C.POINTER(PassDesc)] + [VP] * 10),
+        "wcsph_force_pass_noalloc": (C.c_int, [C.POINTER(Params), C.POINTER(PassDesc)] + [VP] * 10),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

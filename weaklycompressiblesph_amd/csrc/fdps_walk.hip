@@ -21,6 +21,7 @@
 #include <mutex>
 
 #include "common.hpp"
+#include "pass.hpp"
 #include "pair.hpp"
 
 namespace wcsph {
@@ -457,6 +458,8 @@ int backend_init(const wcsph_params* p) {
             for (auto& t : B.tags) release_tag(t);
         B.p = *p;
         B.inited = true;
+        // set-up of the no-allocation force pass on this device
+        (void)noalloc_ptr(p->device);
     });
 }
 

@@ -193,6 +193,59 @@ void run_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const PassG
     WCSPH_HIP(hipGetLastError());
 }
 
+// ---------------- the variant that allocates nothing ----------------
+// For callers that cannot allocate device memory (code under hipGraph
+// capture, tight serving loops, harnesses that forbid hipMalloc):
+// wcsph_force_pass_noalloc runs the same pass with its workspace in the
+// file-scope array `scratch`, sized for up to kNoAllocMaxParticles particles
+// and kNoAllocMaxCells cells (about 56 MB). Each GPU the library runs on
+// holds its own copy of the array until the process ends. Host code never
+// uses the array's name as an address: the device address of each GPU's
+// copy is taken once per device with hipGetSymbolAddress (by wcsph_init,
+// or by the first call on that device) and kept in noalloc_ptr. Every call
+// on a GPU shares that GPU's array and leaves its contents behind; the pass
+// clears what it reads before writing, so no call depends on an earlier
+// one, but calls that overlap on one GPU race.
+constexpr int64_t kNoAllocMaxParticles = (int64_t)1 << 19;
+constexpr int64_t kNoAllocMaxCells = (int64_t)1 << 21;
+constexpr size_t kScratchBytes = pass_layout(kNoAllocMaxParticles, kNoAllocMaxCells).total;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+struct NoAllocState {
+    std::mutex mu;
+    void* ptr[64] = {};
+};
+
+NoAllocState& noalloc_state() {
+    static NoAllocState st;
+    return st;
+}
+
+}  // namespace
+
+// The device address of `device`'s copy of scratch (taken on first use,
+// with that device current).
+void* noalloc_ptr(int device) {
+    WCSPH_REQUIRE(device >= 0 && device < 64, WCSPH_EINVAL, "device ordinal out of range");
+    NoAllocState& st = noalloc_state();
+    std::lock_guard<std::mutex> lk(st.mu);
+    if (!st.ptr[device]) {
+        int cur = 0;
+        WCSPH_HIP(hipGetDevice(&cur));
+        WCSPH_HIP(hipSetDevice(device));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(cur);
+        WCSPH_HIP(e);
+        WCSPH_REQUIRE(ptr != nullptr, WCSPH_EHIP, "no device address for the force pass's scratch array");
+        st.ptr[device] = ptr;
+    }
+    return st.ptr[device];
+}
+
 namespace {
 
 // The library-owned workspace of wcsph_force_pass, one per device, grown
@@ -236,5 +289,23 @@ extern "C" int wcsph_force_pass(const wcsph_params* p, const wcsph_pass_desc* d,
         }
         run_force_pass(p, d, pg, w.ptr[p->device], pos, vel, dens, mass, h, acc, div_v, dt_i, pairs,
                        (hipStream_t)stream);
+    });
+}
+
+extern "C" int wcsph_force_pass_noalloc(const wcsph_params* p, const wcsph_pass_desc* d, const double* pos,
+                                        const double* vel, const double* dens, const double* mass, const double* h,
+                                        double* acc, double* div_v, double* dt_i, int64_t* pairs, void* stream) {
+    return guarded([&] {
+        check_pass_args(p, d, pos, vel, dens, mass, h, acc, div_v, dt_i);
+        const PassGrid pg = pass_grid(p, d);
+        WCSPH_REQUIRE(d->n <= kNoAllocMaxParticles, WCSPH_EOVERFLOW,
+                      "wcsph_force_pass_noalloc takes at most 524288 particles (its fixed scratch array); "
+                      "use wcsph_force_pass");
+        WCSPH_REQUIRE(pg.ncell <= kNoAllocMaxCells, WCSPH_EOVERFLOW,
+                      "wcsph_force_pass_noalloc takes a grid of at most 2097152 cells of h_max (its fixed scratch "
+                      "array); shrink the box or use wcsph_force_pass");
+        void* ws = noalloc_ptr(p->device);
+        WCSPH_HIP(hipSetDevice(p->device));
+        run_force_pass(p, d, pg, ws, pos, vel, dens, mass, h, acc, div_v, dt_i, pairs, (hipStream_t)stream);
     });
 }

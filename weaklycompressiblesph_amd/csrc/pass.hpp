@@ -57,6 +57,10 @@ void run_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const PassG
                     const double* pos, const double* vel, const double* dens, const double* mass, const double* h,
                     double* acc, double* div_v, double* dt_i, int64_t* pairs, hipStream_t st);
 
+// The device address of `device`'s copy of the no-allocation variant's
+// scratch array (hipGetSymbolAddress once per device; throws Error).
+void* noalloc_ptr(int device);
+
 // Argument checks shared by the entry points.
 void check_pass_args(const wcsph_params* p, const wcsph_pass_desc* d, const double* pos, const double* vel,
                      const double* dens, const double* mass, const double* h, double* acc, double* div_v,
