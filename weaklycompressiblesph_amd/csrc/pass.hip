@@ -37,7 +37,7 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t m, uint32_t v) {
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pass_records(const uint32_t* __restrict__ perm,
                                                          const uint32_t* __restrict__ key,
-                                                         const uint2* __restrict__ crange,
+                                                         const uint32_t* __restrict__ start,
                                                          const double* __restrict__ pos,
                                                          const double* __restrict__ vel,
                                                          const double* __restrict__ dens,
@@ -47,8 +47,8 @@ __global__ __launch_bounds__(kBlock) void k_pass_records(const uint32_t* __restr
                                                          Scalars* s) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const uint2 cr = crange[p];
     const uint32_t k = key[perm[p]];
+    const uint2 cr = make_uint2(start[k], start[k + 1]);  // the cell's slot range
     const bool ranked = k != dead_key<D>(g) && cr.y - cr.x <= kRankCap;
     if (k != dead_key<D>(g) && cr.y - cr.x > kRankCap && (uint32_t)p == cr.x) atomicOr(&s->overfull, 1u);
     const uint32_t o = ranked ? ranked_member(perm, cr.x, cr.y, (uint32_t)p) : perm[p];
@@ -138,16 +138,17 @@ void run_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const PassG
     uint32_t* slot = at<uint32_t>(ws, L.slot);
     uint32_t* perm = at<uint32_t>(ws, L.perm);
     uint32_t* id = at<uint32_t>(ws, L.id);
-    uint2* crange = at<uint2>(ws, L.crange);
     uint32_t* count = at<uint32_t>(ws, L.count);
     uint32_t* start = at<uint32_t>(ws, L.start);
-    uint32_t* part = at<uint32_t>(ws, L.part);
+    uint32_t* tile_total = at<uint32_t>(ws, L.tile_total);
+    uint32_t* group_total = at<uint32_t>(ws, L.group_total);
     Scalars* sc = at<Scalars>(ws, L.sc);
     Packed pk{at<float4>(ws, L.px), at<float4>(ws, L.pv), at<float4>(ws, L.pq), at<uint32_t>(ws, L.pcell)};
     const int64_t npart = ceil_div(ncell + 1, kScanTile);
     const dim3 blk(kBlock), grid_n((unsigned)ceil_div(n, kBlock));
     hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, st, sc, (const double*)nullptr, (int64_t)0, 0);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)ceil_div(ncell + 1, kBlock)), blk, 0, st, count, ncell + 1, 0u);
+    const int64_t gsize = scan_group_size(npart), ngroups = ceil_div(npart, gsize);
     Master m{};
     m.pos = const_cast<double*>(pos);
     if (D == 3)
@@ -156,17 +157,18 @@ void run_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const PassG
     else
         hipLaunchKernelGGL(k_bin<2>, grid_n, blk, 0, st, m, (const double*)nullptr, (const double*)nullptr, 0, n, pg.g,
                            key, slot, count, sc);
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)npart), blk, 0, st, count, ncell + 1, part);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanTilesThreads), 0, st, part, npart);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)npart), blk, 0, st, count, ncell + 1, part, start);
-    hipLaunchKernelGGL(k_scatter, grid_n, blk, 0, st, key, slot, start, perm, crange, n);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)ngroups), blk, 0, st, count, ncell + 1, tile_total, group_total,
+                       gsize);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)npart), blk, 0, st, count, ncell + 1, tile_total, group_total, gsize,
+                       start);
+    hipLaunchKernelGGL(k_scatter, grid_n, blk, 0, st, key, slot, start, perm, count, n);
     const double B = p->c0 * p->c0 * p->rho0 / p->gamma;
     const EosConsts e{p->rho0, B, p->gamma};
     if (D == 3)
-        hipLaunchKernelGGL(k_pass_records<3>, grid_n, blk, 0, st, perm, key, crange, pos, vel, dens, mass, h, n, pg.g,
+        hipLaunchKernelGGL(k_pass_records<3>, grid_n, blk, 0, st, perm, key, start, pos, vel, dens, mass, h, n, pg.g,
                            e, pk, id, sc);
     else
-        hipLaunchKernelGGL(k_pass_records<2>, grid_n, blk, 0, st, perm, key, crange, pos, vel, dens, mass, h, n, pg.g,
+        hipLaunchKernelGGL(k_pass_records<2>, grid_n, blk, 0, st, perm, key, start, pos, vel, dens, mass, h, n, pg.g,
                            e, pk, id, sc);
     NFArgs a{};
     a.pk = pk;

@@ -12,7 +12,7 @@ namespace wcsph {
 
 // Byte offsets of the pass's workspace arrays (256-byte aligned).
 struct PassLayout {
-    size_t key, slot, perm, id, crange, px, pv, pq, pcell, count, start, part, sc, total;
+    size_t key, slot, perm, id, px, pv, pq, pcell, count, start, tile_total, group_total, sc, total;
 };
 
 // The cell grid of a pass and its size, from the caller's description
@@ -35,7 +35,6 @@ constexpr PassLayout pass_layout(int64_t n, int64_t ncell) {
     L.slot = take(4 * (size_t)n);
     L.perm = take(4 * (size_t)n);
     L.id = take(4 * (size_t)n);
-    L.crange = take(8 * (size_t)n);
     L.px = take(16 * (size_t)n);
     L.pv = take(16 * (size_t)n);
     L.pq = take(16 * (size_t)n);
@@ -43,7 +42,8 @@ constexpr PassLayout pass_layout(int64_t n, int64_t ncell) {
     // one bucket past the last cell: the dead bucket of non-finite positions
     L.count = take(4 * (size_t)(ncell + 1));
     L.start = take(4 * (size_t)(ncell + 2));
-    L.part = take(4 * (size_t)((ncell + 1 + kScanTile - 1) / kScanTile));
+    L.tile_total = take(4 * (size_t)((ncell + 1 + kScanTile - 1) / kScanTile));
+    L.group_total = take(4 * (size_t)kScanMaxGroups);
     L.sc = take(sizeof(Scalars));
     L.total = off;
     return L;

@@ -50,10 +50,10 @@ struct wcsph_sim {
     double *pr64 = nullptr, *c64 = nullptr;  // FP64 path only
     Mixed mx{};                               // mixed path only
     MixedConsts mc{};
-    uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr,
-             *part = nullptr;
+    uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr;
+    uint32_t *tile_total = nullptr, *group_total = nullptr;  // scan tile and tile-group totals
+    int64_t ngroups = 0;
     uint32_t *nbr = nullptr, *cnt = nullptr;
-    uint2* crange = nullptr;  // cell slot range of each slot (counting sort)
     Scalars* sc = nullptr;
     Partials P{};
     double* diag_part = nullptr;
@@ -160,8 +160,10 @@ static void alloc_grid(wcsph_sim* s, const double* lo, double L, const int32_t* 
     s->npart = ceil_div(ncell + 1, kScanTile);
     s->count = s->alloc<uint32_t>(ncell + 1);
     s->start = s->alloc<uint32_t>(ncell + 2);
-    s->part = s->alloc<uint32_t>(s->npart);
-    // the scan clears count after reading it, ready for the next pass
+    s->ngroups = ceil_div(s->npart, scan_group_size(s->npart));
+    s->tile_total = s->alloc<uint32_t>(s->npart);
+    s->group_total = s->alloc<uint32_t>(s->ngroups);
+    // the scatter clears the counts the scan consumed, ready for the next pass
     WCSPH_HIP(hipMemsetAsync(s->count, 0, sizeof(uint32_t) * (ncell + 1), s->st));
 }
 
@@ -228,12 +230,13 @@ static void bin_and_reorder(wcsph_sim* s, bool step) {
     else
         hipLaunchKernelGGL(k_bin<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src, s->acc, s->divv, kick, n, s->grid,
                            s->key, s->slot, s->count, s->sc);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell + 1, s->part);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanTilesThreads), 0, s->st, s->part, s->npart);
-    hipLaunchKernelGGL(k_scan_apply, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell + 1, s->part,
-                       s->start);
-    hipLaunchKernelGGL(k_scatter, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->key, s->slot, s->start, s->perm,
-                       s->crange, n);
+    const int64_t gsize = scan_group_size(s->npart);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(s->ngroups), dim3(kBlock), 0, s->st, s->count, s->ncell + 1, s->tile_total,
+                       s->group_total, gsize);
+    hipLaunchKernelGGL(k_scan_apply, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell + 1, s->tile_total,
+                       s->group_total, gsize, s->start);
+    hipLaunchKernelGGL(k_scatter, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->key, s->slot, s->start, s->perm, s->count,
+                       n);
     const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
     EosConsts e{s->p.rho0, B, s->p.gamma};
     const bool mixed = s->cfg.precision == WCSPH_PREC_MIXED;
@@ -242,20 +245,20 @@ static void bin_and_reorder(wcsph_sim* s, bool step) {
     const dim3 grid(s->nb(n)), block(kBlock);
     if (s->D == 3) {
         if (mixed)
-            hipLaunchKernelGGL((k_reorder<3, MixedExtra<3>>), grid, block, 0, s->st, s->perm, s->key, s->crange, src,
+            hipLaunchKernelGGL((k_reorder<3, MixedExtra<3>>), grid, block, 0, s->st, s->perm, s->key, s->start, src,
                                dst, s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale,
                                MixedExtra<3>{s->mx, s->mc}, n);
         else
-            hipLaunchKernelGGL((k_reorder<3, NoExtra>), grid, block, 0, s->st, s->perm, s->key, s->crange, src, dst,
+            hipLaunchKernelGGL((k_reorder<3, NoExtra>), grid, block, 0, s->st, s->perm, s->key, s->start, src, dst,
                                s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale, NoExtra{},
                                n);
     } else {
         if (mixed)
-            hipLaunchKernelGGL((k_reorder<2, MixedExtra<2>>), grid, block, 0, s->st, s->perm, s->key, s->crange, src,
+            hipLaunchKernelGGL((k_reorder<2, MixedExtra<2>>), grid, block, 0, s->st, s->perm, s->key, s->start, src,
                                dst, s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale,
                                MixedExtra<2>{s->mx, s->mc}, n);
         else
-            hipLaunchKernelGGL((k_reorder<2, NoExtra>), grid, block, 0, s->st, s->perm, s->key, s->crange, src, dst,
+            hipLaunchKernelGGL((k_reorder<2, NoExtra>), grid, block, 0, s->st, s->perm, s->key, s->start, src, dst,
                                s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale, NoExtra{},
                                n);
     }
@@ -645,7 +648,6 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         s->key = s->alloc<uint32_t>(n);
         s->slot = s->alloc<uint32_t>(n);
         s->perm = s->alloc<uint32_t>(n);
-        s->crange = s->alloc<uint2>(n);
         s->sc = s->alloc<Scalars>(1);
         s->P.nblk = s->nb(n);
         s->P.pairs = s->alloc<unsigned long long>(s->P.nblk);

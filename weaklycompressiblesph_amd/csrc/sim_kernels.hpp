@@ -243,9 +243,7 @@ __global__ void k_bin(Master m, const double* __restrict__ acc, const double* __
 }
 
 // Exclusive scan of count[0..m) into start[0..m], start[m] = total, over
-// tiles of 1024 elements (256 threads x 4): k_scan_reduce writes each tile's
-// total, k_scan_tiles scans the totals in place (one block), k_scan_apply
-// finishes each tile.
+// tiles of 1024 elements (256 threads x 4).
 constexpr int kScanTile = 1024;
 
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
@@ -276,49 +274,70 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return before + incl - v;
 }
 
-static __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t m,
-                                                         uint32_t* __restrict__ part) {
-    __shared__ uint32_t sh[kBlock / 64];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
-    uint32_t v = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v += (base + k < m) ? in[base + k] : 0u;
+// Two-pass exclusive scan of the cell counts, over tiles of kScanTile
+// counts gathered into groups of `gsize` tiles (gsize chosen by the host
+// so there are at most kScanMaxGroups groups). k_scan_reduce (one block
+// per group, one wave per tile in turn) writes every tile's total and the
+// group's total; k_scan_apply gives each tile its offset as the sum of the
+// groups before it and of the tiles before it in its own group (at most
+// kScanMaxGroups + gsize values, read by the block's threads together),
+// then scans the tile. No atomics: same-address atomics from every XCD
+// serialise far from the cores.
+constexpr int kScanMaxGroups = 1024;
+
+__host__ __device__ inline int64_t scan_group_size(int64_t ntiles) {
+    const int64_t g = (ntiles + kScanMaxGroups - 1) / kScanMaxGroups;
+    return g < 16 ? 16 : g;
+}
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* sh) {
     uint32_t total;
     block_exclusive_scan(v, sh, total);
-    if (threadIdx.x == 0) part[blockIdx.x] = total;
+    return total;
 }
 
-constexpr int kScanTilesThreads = 1024;
-static __global__ __launch_bounds__(kScanTilesThreads) void k_scan_tiles(uint32_t* __restrict__ part, int64_t ntiles) {
-    __shared__ uint32_t sh[kScanTilesThreads / 64];
-    uint32_t carry = 0;
-    for (int64_t b = 0; b < ntiles; b += 4 * kScanTilesThreads) {
-        const int64_t base = b + threadIdx.x * 4;
-        uint32_t v[4], sum = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            v[k] = (base + k < ntiles) ? part[base + k] : 0u;
-            sum += v[k];
-        }
-        uint32_t total;
-        uint32_t run = carry + block_exclusive_scan(sum, sh, total);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (base + k < ntiles) part[base + k] = run;
-            run += v[k];
-        }
-        carry += total;
-    }
-}
-
-// Last scan phase: each tile scans its own counts from its offset, writes
-// start and clears count for the next binning pass.
-static __global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ count, int64_t m,
-                                                        const uint32_t* __restrict__ part,
-                                                        uint32_t* __restrict__ start) {
+static __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t m,
+                                                                uint32_t* __restrict__ tile_total,
+                                                                uint32_t* __restrict__ group_total, int64_t gsize) {
     __shared__ uint32_t sh[kBlock / 64];
-    const int64_t tile = blockIdx.x;
-    const uint32_t before = part[tile];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t ntiles = (m + kScanTile - 1) / kScanTile;
+    const int64_t t0 = (int64_t)blockIdx.x * gsize, t1 = t0 + gsize < ntiles ? t0 + gsize : ntiles;
+    uint32_t mine = 0;  // this wave's share of the group total (lane 0)
+    for (int64_t t = t0 + wv; t < t1; t += kBlock / 64) {
+        // lane l sums counts [t * kScanTile + 16 l, + 16)
+        const int64_t base = t * kScanTile + lane * 16;
+        uint32_t v = 0;
+        if (base + 16 <= m) {
+            const uint4* p = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint4 q = p[k];
+                v += q.x + q.y + q.z + q.w;
+            }
+        } else {
+            for (int k = 0; k < 16; ++k) v += base + k < m ? in[base + k] : 0u;
+        }
+        for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+        if (lane == 0) {
+            tile_total[t] = v;
+            mine += v;
+        }
+    }
+    const uint32_t g = block_sum(lane == 0 ? mine : 0u, sh);
+    if (threadIdx.x == 0) group_total[blockIdx.x] = g;
+}
+
+static __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ count, int64_t m,
+                                                               const uint32_t* __restrict__ tile_total,
+                                                               const uint32_t* __restrict__ group_total,
+                                                               int64_t gsize, uint32_t* __restrict__ start) {
+    __shared__ uint32_t sh[kBlock / 64];
+    const int64_t tile = blockIdx.x, g = tile / gsize;
+    uint32_t pre = 0;
+    for (int64_t k = threadIdx.x; k < g; k += kBlock) pre += group_total[k];
+    for (int64_t k = g * gsize + threadIdx.x; k < tile; k += kBlock) pre += tile_total[k];
+    const uint32_t offset = block_sum(pre, sh);
     const int64_t base = tile * kScanTile + threadIdx.x * 4;
     uint32_t v[4], sum = 0;
 #pragma unroll
@@ -327,30 +346,26 @@ static __global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restri
         sum += v[k];
     }
     uint32_t total;
-    uint32_t run = before + block_exclusive_scan(sum, sh, total);
+    uint32_t run = offset + block_exclusive_scan(sum, sh, total);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (base + k < m) {
-            start[base + k] = run;
-            count[base + k] = 0u;
-        }
+        if (base + k < m) start[base + k] = run;
         run += v[k];
         if (base + k == m - 1) start[m] = run;
     }
 }
 
-// Counting sort, pass 2: scatter old indices to their cell slots, each
-// with its cell's slot range [s0, s1).
+// Counting sort, pass 2: scatter old indices to their cell slots, and
+// clear the counts the scan has consumed for the next binning pass (only
+// occupied cells were counted, so only they need clearing).
 static __global__ void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
-                          const uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
-                          uint2* __restrict__ crange, int64_t n) {
+                                 const uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
+                                 uint32_t* __restrict__ count, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = key[i];
-    const uint32_t s0 = start[k], s1 = start[k + 1];
-    const uint32_t d = s0 + slot[i];
-    perm[d] = (uint32_t)i;
-    crange[d] = make_uint2(s0, s1);
+    perm[start[k] + slot[i]] = (uint32_t)i;
+    count[k] = 0u;
 }
 
 struct EosConsts {
@@ -441,7 +456,7 @@ struct Packed {  // FP32 force records in cell order
 template <int D, class Extra>
 __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__ perm,
                                                     const uint32_t* __restrict__ key,
-                                                    const uint2* __restrict__ crange, Master src, Master dst,
+                                                    const uint32_t* __restrict__ start, Master src, Master dst,
                                                     const double* __restrict__ acc_old, Scalars* s, int step,
                                                     int keep_vel, Packed pk, Grid g, double* __restrict__ pr64,
                                                     double* __restrict__ c64, EosConsts e, float v_scale,
@@ -452,8 +467,8 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
     int c[3] = {0, 0, 0};
     double rho = 0.0, m = 0.0, pr = 0.0, cs = 0.0;
     if (live) {
-        const uint2 cr = crange[p];
         const uint32_t k = key[perm[p]];
+        const uint2 cr = make_uint2(start[k], start[k + 1]);  // the cell's slot range
         // the dead bucket and overfull cells keep the scatter's order (the
         // state has diverged; the flag makes the host API report it)
         const bool ranked = k != dead_key<D>(g) && cr.y - cr.x <= kRankCap;
