@@ -129,7 +129,7 @@ __device__ __forceinline__ void write_mixed_pair(const Mixed& mx, const MixedCon
 
 template <int D>
 struct MixedExtra {  // k_reorder / k_apply_density hook writing the pair records
-    static constexpr bool kFp32Velocities = false;  // the mixed pass reads velocities from its records
+    static constexpr bool kFp32Records = false;  // the mixed pass reads only its pair records and the master state
     Mixed mx;
     MixedConsts mc;
     __device__ void store_pair(int64_t p, bool live, const float* xl, int cx, const float* vf, double rho,
@@ -157,9 +157,8 @@ __device__ __forceinline__ h2 hfma(h2 a, h2 b, h2 c) { return __builtin_elementw
 
 struct NFMixedArgs {
     Mixed mx;
-    const float4* px;  // FP32 cell-local positions (row windows of i)
-    const float4* pq;  // FP32 rho, P/rho^2, c, h (outputs of i)
-    const uint32_t* pcell;
+    const double* pos;   // master FP64 positions, cell order (row windows of i)
+    const double* dens;  // master FP64 densities, cell order (epilogue of i)
     const uint32_t* start;
     Grid g;
     int64_t n;
@@ -169,6 +168,7 @@ struct NFMixedArgs {
     h2 k_eps, k_Ls;
     float malpha, beta;  // -2 alpha, 2 beta
     double c0, nu, c_cfl, inv_rho0;
+    float gm1;           // gamma - 1 of the Tait law (sound speed of i in the epilogue)
     double* acc;
     double* divv;
     double* dti;
@@ -253,8 +253,19 @@ void k_nforce_mixed(NFMixedArgs a) {
     const bool failed = sim_failed(a.s);
     const bool live = i < a.n && !failed;
     const int64_t ii = live ? i : 0;
-    const float4 xf = a.px[ii];
-    const float xi[3] = {xf.x, xf.y, xf.z};
+    // the cell and FP32 cell-local coordinates the binning pass gave i (the
+    // same FP64 arithmetic on the same position)
+    int c[3] = {0, 0, 0};
+    float xi[3] = {0.f, 0.f, 0.f};
+    {
+        double x[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) x[d] = a.pos[(int64_t)d * a.n + ii];
+        bool esc = false;
+        cell_of<D>(a.g, x, c, esc);
+#pragma unroll
+        for (int d = 0; d < D; ++d) xi[d] = cell_local(x[d], a.g.lo[d], c[d], a.g.L);
+    }
     const bool odd = (ii & 1) != 0;
     const PosRec16 Pi = a.mx.pos[ii >> 1];
     const VelRec16 Vi = a.mx.vel[ii >> 1];
@@ -273,8 +284,6 @@ void k_nforce_mixed(NFMixedArgs a) {
     const h2 zero2 = hsplat(0.f), tiny2 = hsplat(1.0e-6f), sixteen2 = hsplat(16.f);
     const _Float16 four = (_Float16)4.f;
     const float Lf = a.g.Lf, invLf = 1.0f / Lf;
-    int c[3] = {0, 0, 0};
-    key_cell<D>(a.g, a.pcell[ii], c);
     const float hc = 2.f * a.h * 1.0001f, hc2 = hc * hc;
     float accp[3] = {0.f, 0.f, 0.f}, accv[3] = {0.f, 0.f, 0.f}, divv = 0.f;
     h2 vsig2 = zero2;
@@ -456,15 +465,18 @@ void k_nforce_mixed(NFMixedArgs a) {
     double cand = failed ? 0.0 : 1.0e30;  // a diverged state stops the clock
     if (live) {
         const int64_t n = a.n;
-        const float4 qi4 = a.pq[i];  // FP32 rho, P/rho^2, c, h
+        // FP32 density and its Tait sound speed, c^2 = max(c0^2 (rho/rho0)^(gamma-1), 1e-16)
+        // (src/EoS.h:86)
+        const float rho = (float)a.dens[i], r = (float)(a.dens[i] * a.inv_rho0);
+        const float cs = sqrtf(fmaxf((float)(a.c0 * a.c0) * powf(r, a.gm1), 1.0e-16f));
         const double c0 = a.c0, h = a.h;
         const double ks = c0 * c0 / h, kv = 4.0 * a.nu * c0 / (h * h);
         double ad[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) ad[d] = ks * accp[d] + kv * accv[d];
-        a.divv[i] = -(c0 / h) * divv * (1.0 / ((double)qi4.x * (double)a.inv_rho0));
+        a.divv[i] = -(c0 / h) * divv * (1.0 / ((double)rho * a.inv_rho0));
         const float vs = fmaxf((float)vsig2.x, (float)vsig2.y);
-        const double dti = a.c_cfl * h / ((double)qi4.z + (double)vs * c0);
+        const double dti = a.c_cfl * h / ((double)cs + (double)vs * c0);
         a.dti[i] = dti;
         if (a.fused_final) cand = final_particle<D>(a.fin, a.s, i, n, ad, dti);
 #pragma unroll

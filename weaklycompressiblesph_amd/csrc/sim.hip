@@ -46,7 +46,8 @@ struct wcsph_sim {
     int R = 2;      // cells per support length
     int Q = 16;     // LDS queue depth of the fused pass
     float hmax = 0.f;
-    bool uniform_h = false;  // all smoothing lengths equal (fixed in time)
+    bool uniform_h = false;  // all smoothing lengths equal in FP32 (fixed in time)
+    double h_const = 0.0;    // > 0: all smoothing lengths are exactly this (kept in both buffers, not permuted)
     double *pr64 = nullptr, *c64 = nullptr;  // FP64 path only
     Mixed mx{};                               // mixed path only
     MixedConsts mc{};
@@ -137,7 +138,12 @@ static void scan_h(wcsph_sim* s, const double* h) {
     double hmax = 0.0;
     for (int64_t i = 0; i < n; ++i) hmax = std::max(hmax, (double)(float)h[i]);
     s->uniform_h = true;
-    for (int64_t i = 0; i < n; ++i) s->uniform_h = s->uniform_h && (float)h[i] == (float)h[0];
+    bool same = true;
+    for (int64_t i = 0; i < n; ++i) {
+        s->uniform_h = s->uniform_h && (float)h[i] == (float)h[0];
+        same = same && h[i] == h[0];
+    }
+    s->h_const = same ? h[0] : 0.0;
     WCSPH_REQUIRE(hmax > 0.0, WCSPH_EINVAL, "smoothing lengths must be positive");
     s->hmax = (float)hmax;
 }
@@ -247,20 +253,20 @@ static void bin_and_reorder(wcsph_sim* s, bool step) {
         if (mixed)
             hipLaunchKernelGGL((k_reorder<3, MixedExtra<3>>), grid, block, 0, s->st, s->perm, s->key, s->start, src,
                                dst, s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale,
-                               MixedExtra<3>{s->mx, s->mc}, n);
+                               s->h_const, MixedExtra<3>{s->mx, s->mc}, n);
         else
             hipLaunchKernelGGL((k_reorder<3, NoExtra>), grid, block, 0, s->st, s->perm, s->key, s->start, src, dst,
-                               s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale, NoExtra{},
-                               n);
+                               s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale, s->h_const,
+                               NoExtra{}, n);
     } else {
         if (mixed)
             hipLaunchKernelGGL((k_reorder<2, MixedExtra<2>>), grid, block, 0, s->st, s->perm, s->key, s->start, src,
                                dst, s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale,
-                               MixedExtra<2>{s->mx, s->mc}, n);
+                               s->h_const, MixedExtra<2>{s->mx, s->mc}, n);
         else
             hipLaunchKernelGGL((k_reorder<2, NoExtra>), grid, block, 0, s->st, s->perm, s->key, s->start, src, dst,
-                               s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale, NoExtra{},
-                               n);
+                               s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale, s->h_const,
+                               NoExtra{}, n);
     }
     s->cur = 1 - s->cur;
 }
@@ -285,6 +291,43 @@ static PairConsts<T> pair_consts(const wcsph_sim* s) {
     k.cnorm = (T)(s->D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI);
     k.vsig_mode = s->p.vsig_mode;
     return k;
+}
+
+// The FP32 force records of the current state (cell order) from the master
+// state, as k_reorder writes them for the FP32 and FP64 paths: for the
+// neighbour lists of the mixed path, which does not keep them.
+template <int D>
+__global__ void k_fp32_records(Master m, Packed pk, Grid g, EosConsts e, int64_t n) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double x[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = m.pos[d * n + p];
+    int c[3] = {0, 0, 0};
+    bool esc = false;
+    cell_of<D>(g, x, c, esc);
+    float xl[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        xl[d] = cell_local(x[d], g.lo[d], c[d], g.L);
+        vf[d] = (float)m.vel[d * n + p];
+    }
+    double pr, cs;
+    tait(e, m.dens[p], pr, cs);
+    pk.px[p] = make_float4(xl[0], xl[1], xl[2], (float)c[0]);
+    pk.pv[p] = make_float4(vf[0], vf[1], vf[2], (float)m.mass[p]);
+    pk.pq[p] = make_float4((float)m.dens[p], (float)pr, (float)cs, (float)m.h[p]);
+    pk.pcell[p] = cell_key<D>(g, c);
+}
+
+static void fp32_records(wcsph_sim* s) {
+    const EosConsts e{s->p.rho0, s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma, s->p.gamma};
+    if (s->D == 3)
+        hipLaunchKernelGGL(k_fp32_records<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, s->m[s->cur], s->pk, s->grid,
+                           e, s->n);
+    else
+        hipLaunchKernelGGL(k_fp32_records<2>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, s->m[s->cur], s->pk, s->grid,
+                           e, s->n);
 }
 
 static void ensure_lists(wcsph_sim* s) {
@@ -370,9 +413,9 @@ template <int D, int R, int VS, int BT>
 static void launch_nforce_mixed(wcsph_sim* s) {
     NFMixedArgs a;
     a.mx = s->mx;
-    a.px = s->pk.px;
-    a.pq = s->pk.pq;
-    a.pcell = s->pk.pcell;
+    a.pos = s->m[s->cur].pos;
+    a.dens = s->m[s->cur].dens;
+    a.gm1 = (float)(s->p.gamma - 1.0);
     a.start = s->start;
     a.g = s->grid;
     a.n = s->n;
@@ -703,6 +746,8 @@ int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const d
         WCSPH_HIP(hipMemcpy(m.dens, dens, sizeof(double) * n, hipMemcpyHostToDevice));
         WCSPH_HIP(hipMemcpy(m.mass, mass, sizeof(double) * n, hipMemcpyHostToDevice));
         WCSPH_HIP(hipMemcpy(m.h, h, sizeof(double) * n, hipMemcpyHostToDevice));
+        if (s->h_const > 0.0)  // the reorder does not permute a constant h: both buffers hold it
+            WCSPH_HIP(hipMemcpy(s->m[1 - s->cur].h, h, sizeof(double) * n, hipMemcpyHostToDevice));
         WCSPH_HIP(hipMemcpy(m.type, t8.data(), n, hipMemcpyHostToDevice));
         WCSPH_HIP(hipMemcpy(m.id, id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
         s->uploaded = true;
@@ -916,7 +961,9 @@ int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t 
         WCSPH_REQUIRE(s && s->forced && counts, WCSPH_ESTATE, "no force pass yet or null counts");
         WCSPH_REQUIRE(!lists || cap > 0, WCSPH_EINVAL, "cap must be positive with lists");
         if (s->cfg.precision != WCSPH_PREC_FP64) {
-            // the fused pass keeps no lists: rebuild them for the current cell order
+            // the fused pass keeps no lists: rebuild them for the current cell
+            // order (the mixed path keeps no FP32 records either: write them)
+            if (s->cfg.precision == WCSPH_PREC_MIXED) fp32_records(s);
             hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
             neigh_lists(s, 0);
             WCSPH_HIP(hipGetLastError());
@@ -1241,6 +1288,7 @@ int wcsph_sim_restore(const char* path, int32_t device, wcsph_sim** out) {
         put(m.dens, hs.dens.data(), sizeof(double) * n);
         put(m.mass, hs.mass.data(), sizeof(double) * n);
         put(m.h, hs.h.data(), sizeof(double) * n);
+        if (s->h_const > 0.0) put(s->m[1 - s->cur].h, hs.h.data(), sizeof(double) * n);
         put(s->acc, hs.acc.data(), sizeof(double) * D * n);
         put(s->divv, hs.divv.data(), sizeof(double) * n);
         put(s->dti, hs.dti.data(), sizeof(double) * n);

@@ -460,7 +460,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
                                                     const double* __restrict__ acc_old, Scalars* s, int step,
                                                     int keep_vel, Packed pk, Grid g, double* __restrict__ pr64,
                                                     double* __restrict__ c64, EosConsts e, float v_scale,
-                                                    Extra extra, int64_t n) {
+                                                    double h_const, Extra extra, int64_t n) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = p < n;
     float xl[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
@@ -496,17 +496,21 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
         }
         rho = src.dens[o];
         m = src.mass[o];
-        const double h = src.h[o];
+        // h_const > 0: every smoothing length is h_const, in both buffers,
+        // so the permutation leaves h as it is
+        const double h = h_const > 0.0 ? h_const : src.h[o];
         dst.dens[p] = rho;
         dst.mass[p] = m;
-        dst.h[p] = h;
+        if (!(h_const > 0.0)) dst.h[p] = h;
         dst.type[p] = src.type[o];
         dst.id[p] = src.id[o];
         tait(e, rho, pr, cs);
-        pk.px[p] = make_float4(xl[0], xl[1], xl[2], (float)c[0]);
-        if (Extra::kFp32Velocities) pk.pv[p] = make_float4(vf[0], vf[1], vf[2], (float)m);
-        pk.pq[p] = make_float4((float)rho, (float)pr, (float)cs, (float)h);
-        pk.pcell[p] = k;
+        if (Extra::kFp32Records) {
+            pk.px[p] = make_float4(xl[0], xl[1], xl[2], (float)c[0]);
+            pk.pv[p] = make_float4(vf[0], vf[1], vf[2], (float)m);
+            pk.pq[p] = make_float4((float)rho, (float)pr, (float)cs, (float)h);
+            pk.pcell[p] = k;
+        }
         if (pr64) {
             pr64[p] = pr;
             c64[p] = cs;
@@ -516,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
 }
 
 struct NoExtra {  // FP32 and FP64 paths: the FP32 records are all there is
-    static constexpr bool kFp32Velocities = true;
+    static constexpr bool kFp32Records = true;
     __device__ void store_pair(int64_t, bool, const float*, int, const float*, double, double, double,
                                double) const {}
     __device__ void state(int64_t, double, double, double) const {}
@@ -1055,10 +1059,14 @@ __global__ __launch_bounds__(kBlock) void k_shepard(Master m, Packed pk, const u
     double norm = cnorm;  // C_D / H^D (src/kernel.h:42-48)
 #pragma unroll
     for (int d = 0; d < D; ++d) norm *= invH;
-    const float4 xf = pk.px[i];
-    const float xl[3] = {xf.x, xf.y, xf.z};
-    int c[3];
-    key_cell<D>(g, pk.pcell[i], c);
+    // the cell and cell-local coordinates the binning pass gave i (the same
+    // FP64 arithmetic on the same position)
+    int c[3] = {0, 0, 0};
+    bool esc = false;
+    cell_of<D>(g, xi, c, esc);
+    float xl[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < D; ++d) xl[d] = cell_local(xi[d], g.lo[d], c[d], g.L);
     const float hc = (float)H * 1.0001f;
     Stencil<D, R> st;
     st.k = -1;
@@ -1100,8 +1108,10 @@ __global__ void k_apply_density(Master m, Packed pk, const double* __restrict__ 
     m.dens[p] = rho;
     double pr, cs;
     tait(e, rho, pr, cs);
-    const float4 q = pk.pq[p];
-    pk.pq[p] = make_float4((float)rho, (float)pr, (float)cs, q.w);
+    if (Extra::kFp32Records) {
+        const float4 q = pk.pq[p];
+        pk.pq[p] = make_float4((float)rho, (float)pr, (float)cs, q.w);
+    }
     if (pr64) {
         pr64[p] = pr;
         c64[p] = cs;
