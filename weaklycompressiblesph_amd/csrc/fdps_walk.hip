@@ -3,22 +3,34 @@
 // Replaces GPU_WC::HydrDispatchKernel / HydrRetrieveKernel
 // (src/kernel.cu:289-409). Differences by design:
 //  * records are packed in parallel into pinned SoA blocks, one block per
-//    chunk of about 1/8 of the batch; each chunk's upload and kernel are
-//    queued on the tag's stream as soon as it is packed, so packing the
-//    next chunk overlaps the transfer and the force of this one (the
-//    reference packs serially into AoS half records and issues four
-//    synchronous copies, src/kernel.cu:317-373);
+//    chunk of about 1/8 of the batch; each chunk's upload is queued on a
+//    copy stream as soon as it is packed and its kernel on one of two
+//    kernel streams (even and odd chunks alternate), so packing the next
+//    chunk overlaps the transfer and the force of this one, and a chunk's
+//    tail of long walks overlaps the next chunk's kernel (the reference
+//    packs serially into AoS half records and issues four synchronous
+//    copies, src/kernel.cu:317-373);
 //  * positions and velocities are shipped in FP32 relative to the walk's
 //    first i particle, the subtraction done in FP64 on the host (the
 //    reference shifts by the walk's centre of mass and mean velocity,
 //    src/kernel.cu:318-327, and stores FP16); pair differences are FP32;
-//  * one workgroup per walk stages j tiles in LDS and each lane owns one i;
-//    per-particle h and m are honoured (the reference assumes the first
-//    particle's h and m for every pair, src/kernel.cu:303-304,400-401);
-//  * the result copy is queued behind the last chunk's kernel, so retrieve
-//    only waits and unpacks.
+//  * one workgroup per walk (long walks cut into pieces along the j list,
+//    launched longest first) stages j tiles in LDS; each i is owned by
+//    256 / (number of i in the walk) lanes that split its j list; each lane
+//    lists its neighbours by the support test alone, then runs the pair
+//    body over the list, and all partial sums are added in a fixed order
+//    (the reference gives one thread per i and evaluates every candidate,
+//    src/kernel.cu:241-280); per-particle h and m are honoured (the
+//    reference assumes the first particle's h and m for every pair,
+//    src/kernel.cu:303-304,400-401);
+//  * the result copy is queued behind the last kernels of both streams, so
+//    retrieve only waits and unpacks.
+#include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <mutex>
+#include <utility>
+#include <vector>
 
 #include "common.hpp"
 #include "pass.hpp"
@@ -28,14 +40,30 @@ namespace wcsph {
 
 namespace {
 
-constexpr int kWalkThreads = 128;
-constexpr int kWalkTile = 128;
+constexpr int kWalkThreads = 256;
+constexpr int kWalkTile = 256;  // j records per LDS tile (list entries are one byte)
+constexpr int kWalkCap = 64;    // neighbours listed per lane before the pair body runs
+constexpr int kWalkScan = 4;    // candidates tested per scan step
+constexpr int kWalkPiece = 128; // serial candidates per lane above which a walk is cut into pieces
+constexpr int kWalkMinPieceJ = 64;  // fewest j candidates of a piece
 constexpr int kMaxTags = 8;
 
 template <int D>
 struct WalkArgs {
     const int32_t* ioff;  // [n_walk+1], chunk-local
     const int32_t* joff;  // [n_walk+1], chunk-local
+    // work items, longest first: walk, j range [jb, je) (chunk-local), and
+    // where the sums go (-1: final results in res; else the float offset of
+    // this piece's partial sums in part)
+    const int32_t* it_walk;
+    const int32_t* it_jb;
+    const int32_t* it_je;
+    const int32_t* it_out;
+    // walks cut into pieces: walk, float offset of its partials, pieces
+    const int32_t* sp_walk;
+    const int32_t* sp_base;
+    const int32_t* sp_np;
+    float* part;          // partial sums of pieces: [piece][i of walk][D+2]
     const float* ix;      // [D][ni]: x - x0 of the walk
     const float* iv;      // [D][ni]: v - v0 of the walk
     const float* iq;      // [5][ni]: h rho pr c nu
@@ -49,17 +77,48 @@ struct WalkArgs {
     float c_cfl;
 };
 
-template <int D>
+// One workgroup of kWalkThreads per work item: a walk, or one piece of a
+// long walk's j list (its sums then go to a partial buffer that
+// k_walk_combine adds up in piece order). A round takes nr <= kWalkThreads
+// of the walk's i particles and splits each one's j list over
+// S = kWalkThreads / nr threads ("parts"): thread t takes i = t % nr and
+// every S-th candidate of each tile from t / nr on. The parts' sums are
+// combined in part order through LDS at the end of the round, so the
+// result does not depend on scheduling. j records are staged in LDS as
+// float4 {x, y, z, h}, {vx, vy, vz, m}, {rho, P/rho^2, c, nu}.
+//
+// The superset lists hold about 7 candidates per neighbour, and the lanes of
+// a wave (64 different i against the same j) seldom agree on which are
+// neighbours, so each lane first scans its candidates with the support test
+// alone and appends the tile index of every neighbour to a list of its own
+// in LDS (kWalkCap entries, one byte each); the pair body then runs over the
+// list, every lane on a neighbour of its own. Neighbours are taken in
+// candidate order, so the sums are those of a plain loop. In v_sig mode 0
+// (src/force.h:47, where v_sig stays 0) nothing outside the support is
+// evaluated; in mode 1 (src/kernel.cu:263) -mu enters v_sig for every
+// candidate during the scan, as the reference kernel takes it over the
+// whole list.
+template <int D, int VS>
 __global__ __launch_bounds__(kWalkThreads) void k_walk_force(WalkArgs<D> a) {
-    __shared__ float sx[D][kWalkTile];
-    __shared__ float sv[D][kWalkTile];
-    __shared__ float sq[6][kWalkTile];
-    const int w = blockIdx.x;
+    __shared__ float4 stage[3][kWalkTile];
+    __shared__ uint8_t lst[kWalkCap][kWalkThreads];  // lane t's list is column t
+    float4* sx = stage[0];
+    float4* sv = stage[1];
+    float4* sq = stage[2];
+    // the parts' sums (acc[D], divv, vsig of each thread) reuse sv and sq
+    static_assert((D + 2) * kWalkThreads * sizeof(float) <= 2 * kWalkTile * sizeof(float4), "red fits sv, sq");
+    float(*red)[kWalkThreads] = reinterpret_cast<float(*)[kWalkThreads]>(&stage[1][0]);
+    const int w = a.it_walk[blockIdx.x];
     const int i0 = a.ioff[w], i1 = a.ioff[w + 1];
-    const int j0 = a.joff[w], j1 = a.joff[w + 1];
+    const int j0 = a.it_jb[blockIdx.x], j1 = a.it_je[blockIdx.x];
+    const int out = a.it_out[blockIdx.x];
+    const int t = (int)threadIdx.x;
     for (int ib = i0; ib < i1; ib += kWalkThreads) {
-        const int i = ib + (int)threadIdx.x;
-        const bool live = i < i1;
+        const int nr = min(kWalkThreads, i1 - ib);  // i particles of this round (block-uniform)
+        const int S = kWalkThreads / nr;             // parts per i
+        const int part = t / nr;
+        const int i = ib + t % nr;
+        const bool live = part < S;
         float xi[D], vi[D];
         IState<float> I{1.f, 1.f, 0.f, 1.f, 0.f};
 #pragma unroll
@@ -81,46 +140,147 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk_force(WalkArgs<D> a) {
         for (int jb = j0; jb < j1; jb += kWalkTile) {
             const int nt = min(kWalkTile, j1 - jb);
             __syncthreads();
-            for (int t = threadIdx.x; t < nt; t += kWalkThreads) {
-                const int j = jb + t;
+            for (int u = t; u < nt; u += kWalkThreads) {
+                const int j = jb + u;
+                float x[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    sx[d][t] = a.jx[d * a.nj + j];
-                    sv[d][t] = a.jv[d * a.nj + j];
+                    x[d] = a.jx[d * a.nj + j];
+                    v[d] = a.jv[d * a.nj + j];
                 }
-#pragma unroll
-                for (int q = 0; q < 6; ++q) sq[q][t] = a.jq[q * a.nj + j];
+                sx[u] = make_float4(x[0], x[1], x[2], a.jq[0 * a.nj + j]);
+                sv[u] = make_float4(v[0], v[1], v[2], a.jq[5 * a.nj + j]);
+                sq[u] = make_float4(a.jq[1 * a.nj + j], a.jq[2 * a.nj + j], a.jq[3 * a.nj + j], a.jq[4 * a.nj + j]);
             }
             __syncthreads();
-            if (!live) continue;
-            for (int t = 0; t < nt; ++t) {
-                float dr[D], dv[D];
-                float r2 = 0.f, drdv = 0.f;
+            // this lane's candidates: every S-th of the tile from its part on
+            int u = live ? part : nt;
+            // the support test of candidate v (appends v to the list; r2 = 0,
+            // the i particle itself, is listed and contributes nothing)
+            auto test = [&](int v, const float4& pj, int& cnt) {
+                float r2 = 0.f;
+                float dr[D];
+                const float pjv[3] = {pj.x, pj.y, pj.z};
 #pragma unroll
                 for (int d = 0; d < D; ++d) {
-                    dr[d] = xi[d] - sx[d][t];
-                    dv[d] = vi[d] - sv[d][t];
+                    dr[d] = xi[d] - pjv[d];
                     r2 += dr[d] * dr[d];
-                    drdv += dr[d] * dv[d];
                 }
-                if (r2 <= 0.f) continue;  // src/force.h:42
-                const float hj = sq[0][t];
-                float mu;
-                const float den = pair_mu(r2, drdv, I, hj, a.k, mu, vsig);
-                const float H = I.h + hj;
-                if (r2 < H * H) {
-                    JState<float> J{hj, sq[1][t], sq[2][t], sq[3][t], sq[4][t], sq[5][t]};
+                if (VS == 1 && r2 > 0.f) {
+                    const float4 vj = sv[v];
+                    const float vjv[3] = {vj.x, vj.y, vj.z};
+                    float drdv = 0.f;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) drdv += dr[d] * (vi[d] - vjv[d]);
+                    float mu;
+                    (void)pair_mu(r2, drdv, I, pj.w, a.k, mu, vsig);
+                }
+                const float H = I.h + pj.w;
+                lst[cnt][t] = (uint8_t)v;  // kept only if counted
+                cnt += r2 < H * H ? 1 : 0;
+            };
+            while (u < nt) {
+                // scan in groups of kWalkScan candidates (their LDS reads in
+                // flight together) until the list is nearly full, then the
+                // last few one by one
+                int cnt = 0;
+                for (; u + (kWalkScan - 1) * S < nt && cnt <= kWalkCap - kWalkScan; u += kWalkScan * S) {
+                    float4 pj[kWalkScan];
+#pragma unroll
+                    for (int q = 0; q < kWalkScan; ++q) pj[q] = sx[u + q * S];
+#pragma unroll
+                    for (int q = 0; q < kWalkScan; ++q) test(u + q * S, pj[q], cnt);
+                }
+                if (u + (kWalkScan - 1) * S >= nt)
+                    for (; u < nt && cnt < kWalkCap; u += S) test(u, sx[u], cnt);
+                // pair body over the list
+                for (int k = 0; k < cnt; ++k) {
+                    const int v = lst[k][t];
+                    const float4 pj = sx[v];
+                    const float4 vj = sv[v];
+                    const float pjv[3] = {pj.x, pj.y, pj.z};
+                    const float vjv[3] = {vj.x, vj.y, vj.z};
+                    float dr[D], dv[D];
+                    float r2 = 0.f, drdv = 0.f;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        dr[d] = xi[d] - pjv[d];
+                        r2 += dr[d] * dr[d];
+                    }
+                    if (r2 <= 0.f) continue;  // src/force.h:42
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        dv[d] = vi[d] - vjv[d];
+                        drdv += dr[d] * dv[d];
+                    }
+                    float mu;
+                    const float den = pair_mu(r2, drdv, I, pj.w, a.k, mu, vsig);  // max: a repeat is harmless
+                    const float4 qj = sq[v];
+                    JState<float> J{pj.w, qj.x, qj.y, qj.z, qj.w, vj.w};
                     pair_force<D>(dr, dv, r2, drdv, den, mu, I, J, a.k, acc, divv);
                 }
             }
         }
-        if (live) {
-            float* r = a.res + (a.i_base + i) * (D + 2);
+        // combine the parts of each i in part order
+        __syncthreads();  // the last tile's sv and sq are read no more
 #pragma unroll
-            for (int d = 0; d < D; ++d) r[d] = acc[d];
-            r[D] = a.c_cfl * I.h / (I.c + vsig);  // src/force.h:56
-            r[D + 1] = divv / I.rho;              // src/force.h:57
+        for (int d = 0; d < D; ++d) red[d][t] = acc[d];
+        red[D][t] = divv;
+        red[D + 1][t] = vsig;
+        __syncthreads();
+        if (t < nr) {
+            for (int p = 1; p < S; ++p) {
+                const int o = t + p * nr;
+#pragma unroll
+                for (int d = 0; d < D; ++d) acc[d] += red[d][o];
+                divv += red[D][o];
+                vsig = fmaxf(vsig, red[D + 1][o]);
+            }
+            if (out < 0) {
+                float* r = a.res + (a.i_base + i) * (D + 2);
+#pragma unroll
+                for (int d = 0; d < D; ++d) r[d] = acc[d];
+                r[D] = a.c_cfl * I.h / (I.c + vsig);  // src/force.h:56
+                r[D + 1] = divv / I.rho;              // src/force.h:57
+            } else {
+                float* r = a.part + out + (int64_t)(i - i0) * (D + 2);
+#pragma unroll
+                for (int d = 0; d < D; ++d) r[d] = acc[d];
+                r[D] = divv;
+                r[D + 1] = vsig;
+            }
         }
+    }
+}
+
+// Results of the walks cut into pieces: the pieces' partial sums of each i
+// added in piece order (one workgroup per such walk).
+template <int D>
+__global__ __launch_bounds__(kWalkThreads) void k_walk_combine(WalkArgs<D> a) {
+    const int s = blockIdx.x;
+    const int w = a.sp_walk[s];
+    const int np = a.sp_np[s];
+    const int i0 = a.ioff[w], n_i = a.ioff[w + 1] - i0;
+    const float* base = a.part + a.sp_base[s];
+    for (int li = (int)threadIdx.x; li < n_i; li += kWalkThreads) {
+        float acc[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d] = 0.f;
+        float divv = 0.f, vsig = 0.f;
+        for (int p = 0; p < np; ++p) {
+            const float* q = base + ((int64_t)p * n_i + li) * (D + 2);
+#pragma unroll
+            for (int d = 0; d < D; ++d) acc[d] += q[d];
+            divv += q[D];
+            vsig = fmaxf(vsig, q[D + 1]);
+        }
+        const int i = i0 + li;
+        const float h = a.iq[0 * a.ni + i], rho = a.iq[1 * a.ni + i], c = a.iq[3 * a.ni + i];
+        float* r = a.res + (a.i_base + i) * (D + 2);
+#pragma unroll
+        for (int d = 0; d < D; ++d) r[d] = acc[d];
+        r[D] = a.c_cfl * h / (c + vsig);  // src/force.h:56
+        r[D + 1] = divv / rho;            // src/force.h:57
     }
 }
 
@@ -132,6 +292,8 @@ struct TagState {
     int dim = 0;
     hipStream_t stream = nullptr;       // kernels and the result copy
     hipStream_t copy_stream = nullptr;  // uploads, so chunk c + 1 moves while chunk c computes
+    hipStream_t stream2 = nullptr;      // odd chunks' kernels, so a chunk's tail overlaps the next chunk
+    hipEvent_t ev_k2 = nullptr;         // stream2's last kernel done
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;       // first upload .. result copy done
     hipEvent_t ev_k[2 * kMaxChunks] = {};                   // around each chunk's kernel
     hipEvent_t ev_up[kMaxChunks] = {};                      // each chunk's upload done
@@ -143,6 +305,8 @@ struct TagState {
     float* h_res = nullptr;
     float* d_res = nullptr;
     size_t res_cap = 0;
+    float* d_part = nullptr;  // partial sums of the pieces of long walks
+    size_t part_cap = 0;
     std::vector<int32_t> ioff;  // offsets of the outstanding batch
     int64_t ni = 0;
     double pack_ms = 0.0, unpack_ms = 0.0;
@@ -167,6 +331,7 @@ void release_tag(TagState& t) {
     if (t.d_in) (void)hipFree(t.d_in);
     if (t.h_res) (void)hipHostFree(t.h_res);
     if (t.d_res) (void)hipFree(t.d_res);
+    if (t.d_part) (void)hipFree(t.d_part);
     (void)hipStreamSynchronize(t.copy_stream);
     (void)hipEventDestroy(t.ev_begin);
     (void)hipEventDestroy(t.ev_end);
@@ -174,6 +339,9 @@ void release_tag(TagState& t) {
     for (auto& e : t.ev_up) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(t.stream);
     (void)hipStreamDestroy(t.copy_stream);
+    (void)hipStreamSynchronize(t.stream2);
+    (void)hipStreamDestroy(t.stream2);
+    (void)hipEventDestroy(t.ev_k2);
     t = TagState{};
 }
 
@@ -181,14 +349,30 @@ size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
 
 // Byte layout of one staged chunk (identical on host and device).
 struct Layout {
-    size_t ioff, joff, ix, iv, iq, jx, jv, jq, total;
+    size_t ioff, joff, it_walk, it_jb, it_je, it_out, sp_walk, sp_base, sp_np, ix, iv, iq, jx, jv, jq, total;
 };
 
-Layout layout(int D, int n_walk, int64_t ni, int64_t nj) {
+// The work items of one chunk: every walk, the long ones cut along their j
+// list into pieces of about kWalkPiece serial candidates per lane.
+struct ChunkPlan {
+    std::vector<int32_t> it_walk, it_jb, it_je, it_out;
+    std::vector<int32_t> sp_walk, sp_base, sp_np;
+    int64_t part_floats = 0;
+};
+
+Layout layout(int D, int n_walk, int64_t ni, int64_t nj, const ChunkPlan& P) {
     Layout L{};
     size_t o = 0;
+    const size_t nit = std::max<size_t>(P.it_walk.size(), 1), nsp = std::max<size_t>(P.sp_walk.size(), 1);
     L.ioff = o; o = align_up(o + sizeof(int32_t) * (n_walk + 1));
     L.joff = o; o = align_up(o + sizeof(int32_t) * (n_walk + 1));
+    L.it_walk = o; o = align_up(o + sizeof(int32_t) * nit);
+    L.it_jb = o; o = align_up(o + sizeof(int32_t) * nit);
+    L.it_je = o; o = align_up(o + sizeof(int32_t) * nit);
+    L.it_out = o; o = align_up(o + sizeof(int32_t) * nit);
+    L.sp_walk = o; o = align_up(o + sizeof(int32_t) * nsp);
+    L.sp_base = o; o = align_up(o + sizeof(int32_t) * nsp);
+    L.sp_np = o; o = align_up(o + sizeof(int32_t) * nsp);
     L.ix = o; o = align_up(o + sizeof(float) * D * ni);
     L.iv = o; o = align_up(o + sizeof(float) * D * ni);
     L.iq = o; o = align_up(o + sizeof(float) * 5 * ni);
@@ -203,10 +387,61 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Work items of walks [w0, w1). A wave's time is its lanes' serial
+// candidate count, the sum over i rounds of ceil(n_j / parts); a walk whose
+// count exceeds kWalkPiece is cut into that many equal j ranges, each its
+// own workgroup writing partial sums, so the longest walks no longer set
+// the chunk's time. Items are launched longest first: a workgroup
+// dispatched early is older, and the oldest waves of a SIMD issue first.
+ChunkPlan plan_chunk(int D, int w0, int w1, const std::vector<int32_t>& ioff, const std::vector<int32_t>& joff) {
+    ChunkPlan P;
+    std::vector<std::pair<int64_t, int32_t>> cost;  // (-cost, item)
+    for (int w = w0; w < w1; ++w) {
+        const int lw = w - w0;
+        const int n_i = ioff[w + 1] - ioff[w], n_j = joff[w + 1] - joff[w];
+        const int jw = joff[w] - joff[w0];
+        if (n_i == 0) continue;
+        int64_t c = 0;
+        for (int r = 0; r < n_i; r += kWalkThreads) {
+            const int parts = kWalkThreads / std::min(kWalkThreads, n_i - r);
+            c += (n_j + parts - 1) / parts;
+        }
+        int np = (int)std::min<int64_t>((c + kWalkPiece - 1) / kWalkPiece, std::max(1, n_j / kWalkMinPieceJ));
+        np = std::max(np, 1);
+        int32_t out = -1;
+        if (np > 1) {
+            WCSPH_REQUIRE(P.part_floats + (int64_t)np * n_i * (D + 2) < INT32_MAX, WCSPH_EOVERFLOW,
+                          "partial sums exceed 2^31 floats");
+            out = (int32_t)P.part_floats;
+            P.sp_walk.push_back(lw);
+            P.sp_base.push_back(out);
+            P.sp_np.push_back(np);
+            P.part_floats += (int64_t)np * n_i * (D + 2);
+        }
+        for (int p = 0; p < np; ++p) {
+            cost.push_back({-(c / np), (int32_t)P.it_walk.size()});
+            P.it_walk.push_back(lw);
+            P.it_jb.push_back(jw + (int32_t)((int64_t)n_j * p / np));
+            P.it_je.push_back(jw + (int32_t)((int64_t)n_j * (p + 1) / np));
+            P.it_out.push_back(np > 1 ? out + p * n_i * (D + 2) : -1);
+        }
+    }
+    std::sort(cost.begin(), cost.end());
+    ChunkPlan S = P;
+    for (size_t k = 0; k < cost.size(); ++k) {
+        const int32_t src = cost[k].second;
+        S.it_walk[k] = P.it_walk[src];
+        S.it_jb[k] = P.it_jb[src];
+        S.it_je[k] = P.it_je[src];
+        S.it_out[k] = P.it_out[src];
+    }
+    return S;
+}
+
 // Pack walks [w0, w1) of the batch into one chunk block at h (SoA, FP32,
 // positions and velocities relative to each walk's first i particle).
 template <int D, class EPI, class EPJ>
-void pack_chunk(char* h, const Layout& L, int w0, int w1, const std::vector<int32_t>& ioff,
+void pack_chunk(char* h, const Layout& L, const ChunkPlan& P, int w0, int w1, const std::vector<int32_t>& ioff,
                 const std::vector<int32_t>& joff, const EPI* const* epi, const EPJ* const* epj) {
     const int nw = w1 - w0;
     const int64_t ni = ioff[w1] - ioff[w0], nj = joff[w1] - joff[w0];
@@ -216,6 +451,16 @@ void pack_chunk(char* h, const Layout& L, int w0, int w1, const std::vector<int3
         lio[w] = ioff[w0 + w] - ioff[w0];
         ljo[w] = joff[w0 + w] - joff[w0];
     }
+    auto put = [h](size_t at, const std::vector<int32_t>& v) {
+        if (!v.empty()) std::memcpy(h + at, v.data(), sizeof(int32_t) * v.size());
+    };
+    put(L.it_walk, P.it_walk);
+    put(L.it_jb, P.it_jb);
+    put(L.it_je, P.it_je);
+    put(L.it_out, P.it_out);
+    put(L.sp_walk, P.sp_walk);
+    put(L.sp_base, P.sp_base);
+    put(L.sp_np, P.sp_np);
     float* ix = (float*)(h + L.ix);
     float* iv = (float*)(h + L.iv);
     float* iq = (float*)(h + L.iq);
@@ -277,6 +522,8 @@ void dispatch(int32_t tag, int32_t n_walk, const EPI* const* epi, const int32_t*
         WCSPH_HIP(hipSetDevice(B.p.device));
         WCSPH_HIP(hipStreamCreateWithFlags(&T.stream, hipStreamNonBlocking));
         WCSPH_HIP(hipStreamCreateWithFlags(&T.copy_stream, hipStreamNonBlocking));
+        WCSPH_HIP(hipStreamCreateWithFlags(&T.stream2, hipStreamNonBlocking));
+        WCSPH_HIP(hipEventCreateWithFlags(&T.ev_k2, hipEventDisableTiming));
         WCSPH_HIP(hipEventCreate(&T.ev_begin));
         WCSPH_HIP(hipEventCreate(&T.ev_end));
         for (auto& e : T.ev_k) WCSPH_HIP(hipEventCreate(&e));
@@ -284,9 +531,11 @@ void dispatch(int32_t tag, int32_t n_walk, const EPI* const* epi, const int32_t*
         T.ready = true;
     }
     WCSPH_HIP(hipSetDevice(B.p.device));
-    // uploads of a dispatch that failed part way may still read the staging
-    // buffer: let them finish before it is packed again
+    // uploads and kernels of a dispatch that failed part way may still read
+    // the staging buffers: let them finish before those are packed again
     WCSPH_HIP(hipStreamSynchronize(T.copy_stream));
+    WCSPH_HIP(hipStreamSynchronize(T.stream));
+    WCSPH_HIP(hipStreamSynchronize(T.stream2));
     std::vector<int32_t> ioff(n_walk + 1), joff(n_walk + 1);
     int64_t ni = 0, nj = 0;
     ioff[0] = joff[0] = 0;
@@ -311,11 +560,24 @@ void dispatch(int32_t tag, int32_t n_walk, const EPI* const* epi, const int32_t*
             if (((int64_t)ioff[w] + joff[w]) * nch >= tot * c) wb[c++] = w;
         for (; c < nch; ++c) wb[c] = n_walk;
     }
+    std::vector<ChunkPlan> plans(nch);
     std::vector<Layout> Ls(nch);
     std::vector<size_t> base(nch + 1, 0);
+    std::vector<int64_t> pbase(nch + 1, 0);
     for (int c = 0; c < nch; ++c) {
-        Ls[c] = layout(D, wb[c + 1] - wb[c], ioff[wb[c + 1]] - ioff[wb[c]], joff[wb[c + 1]] - joff[wb[c]]);
+        plans[c] = plan_chunk(D, wb[c], wb[c + 1], ioff, joff);
+        Ls[c] = layout(D, wb[c + 1] - wb[c], ioff[wb[c + 1]] - ioff[wb[c]], joff[wb[c + 1]] - joff[wb[c]],
+                       plans[c]);
         base[c + 1] = base[c] + Ls[c].total;
+        pbase[c + 1] = pbase[c] + align_up(plans[c].part_floats * sizeof(float)) / sizeof(float);
+    }
+    const size_t part_bytes = std::max<size_t>(sizeof(float) * pbase[nch], 256);
+    if (part_bytes > T.part_cap) {
+        if (T.d_part) WCSPH_HIP(hipFree(T.d_part));
+        T.d_part = nullptr;
+        const size_t cap = part_bytes + part_bytes / 4;
+        WCSPH_HIP(hipMalloc((void**)&T.d_part, cap));
+        T.part_cap = cap;
     }
     const size_t total = std::max<size_t>(base[nch], 256);
     if (total > T.h_in_cap) {
@@ -354,16 +616,26 @@ void dispatch(int32_t tag, int32_t n_walk, const EPI* const* epi, const int32_t*
         const Layout& L = Ls[c];
         char* h = T.h_in + base[c];
         const auto t0 = std::chrono::steady_clock::now();
-        pack_chunk<D>(h, L, w0, w1, ioff, joff, epi, epj);
+        pack_chunk<D>(h, L, plans[c], w0, w1, ioff, joff, epi, epj);
         pack_ms += ms_since(t0);
         // the chunk goes to the device while the next one is packed, and its
         // kernel waits only for its own upload
         WCSPH_HIP(hipMemcpyAsync(T.d_in + base[c], h, L.total, hipMemcpyHostToDevice, T.copy_stream));
         WCSPH_HIP(hipEventRecord(T.ev_up[c], T.copy_stream));
-        WCSPH_HIP(hipStreamWaitEvent(T.stream, T.ev_up[c], 0));
+        // even chunks' kernels on the tag's stream, odd ones on stream2
+        hipStream_t ks = (c & 1) ? T.stream2 : T.stream;
+        WCSPH_HIP(hipStreamWaitEvent(ks, T.ev_up[c], 0));
         char* d = T.d_in + base[c];
         a.ioff = (const int32_t*)(d + L.ioff);
         a.joff = (const int32_t*)(d + L.joff);
+        a.it_walk = (const int32_t*)(d + L.it_walk);
+        a.it_jb = (const int32_t*)(d + L.it_jb);
+        a.it_je = (const int32_t*)(d + L.it_je);
+        a.it_out = (const int32_t*)(d + L.it_out);
+        a.sp_walk = (const int32_t*)(d + L.sp_walk);
+        a.sp_base = (const int32_t*)(d + L.sp_base);
+        a.sp_np = (const int32_t*)(d + L.sp_np);
+        a.part = T.d_part + pbase[c];
         a.ix = (const float*)(d + L.ix);
         a.iv = (const float*)(d + L.iv);
         a.iq = (const float*)(d + L.iq);
@@ -373,11 +645,21 @@ void dispatch(int32_t tag, int32_t n_walk, const EPI* const* epi, const int32_t*
         a.ni = ioff[w1] - ioff[w0];
         a.nj = joff[w1] - joff[w0];
         a.i_base = ioff[w0];
-        WCSPH_HIP(hipEventRecord(T.ev_k[2 * c], T.stream));
-        if (w1 > w0) hipLaunchKernelGGL(k_walk_force<D>, dim3(w1 - w0), dim3(kWalkThreads), 0, T.stream, a);
+        WCSPH_HIP(hipEventRecord(T.ev_k[2 * c], ks));
+        const int nit = (int)plans[c].it_walk.size(), nsp = (int)plans[c].sp_walk.size();
+        if (nit > 0) {
+            if (B.p.vsig_mode)
+                hipLaunchKernelGGL((k_walk_force<D, 1>), dim3(nit), dim3(kWalkThreads), 0, ks, a);
+            else
+                hipLaunchKernelGGL((k_walk_force<D, 0>), dim3(nit), dim3(kWalkThreads), 0, ks, a);
+        }
+        if (nsp > 0) hipLaunchKernelGGL((k_walk_combine<D>), dim3(nsp), dim3(kWalkThreads), 0, ks, a);
         WCSPH_HIP(hipGetLastError());
-        WCSPH_HIP(hipEventRecord(T.ev_k[2 * c + 1], T.stream));
+        WCSPH_HIP(hipEventRecord(T.ev_k[2 * c + 1], ks));
     }
+    // the result copy follows the kernels of both streams
+    WCSPH_HIP(hipEventRecord(T.ev_k2, T.stream2));
+    WCSPH_HIP(hipStreamWaitEvent(T.stream, T.ev_k2, 0));
     if (ni > 0)
         WCSPH_HIP(hipMemcpyAsync(T.h_res, T.d_res, sizeof(float) * (D + 2) * ni, hipMemcpyDeviceToHost,
                                  T.stream));
@@ -429,11 +711,26 @@ int backend_last_timing(int32_t tag, double* out) {
         TagState& T = B.tags[tag];
         WCSPH_REQUIRE(T.ready && T.nchunks > 0, WCSPH_ESTATE, "tag has no batch yet");
         WCSPH_HIP(hipEventSynchronize(T.ev_end));
-        float ms = 0.f, k = 0.f;
+        // kernel time: the union of the chunk kernels' intervals (kernels of
+        // consecutive chunks may overlap on the two streams)
+        std::vector<std::pair<float, float>> iv((size_t)T.nchunks);
         for (int c = 0; c < T.nchunks; ++c) {
-            WCSPH_HIP(hipEventElapsedTime(&ms, T.ev_k[2 * c], T.ev_k[2 * c + 1]));
-            k += ms;
+            WCSPH_HIP(hipEventElapsedTime(&iv[c].first, T.ev_begin, T.ev_k[2 * c]));
+            WCSPH_HIP(hipEventElapsedTime(&iv[c].second, T.ev_begin, T.ev_k[2 * c + 1]));
         }
+        std::sort(iv.begin(), iv.end());
+        float k = 0.f, cur_a = 0.f, cur_b = -1.f;
+        for (const auto& x : iv) {
+            if (x.first > cur_b) {
+                if (cur_b > cur_a) k += cur_b - cur_a;
+                cur_a = x.first;
+                cur_b = x.second;
+            } else {
+                cur_b = std::max(cur_b, x.second);
+            }
+        }
+        if (cur_b > cur_a) k += cur_b - cur_a;
+        float ms = 0.f;
         WCSPH_HIP(hipEventElapsedTime(&ms, T.ev_begin, T.ev_end));
         out[0] = T.pack_ms;
         out[1] = ms;
