@@ -12,18 +12,23 @@ value = neighbour pair interactions evaluated in the timed steps / wall time
 of those steps (replayed from the library's two-step hipGraphs, as users
 step), for the mixed-precision path (packed FP16 pair geometry, FP32
 density, pressure and accumulation, FP64 integration); the FP32 plain path
-runs the same steps beside it ("paths", "mixed_over_fp32"). The default run
-is the configuration's 2,000 steps: 1,000 warm-up steps, then steps
-1,001-2,000 timed. The roofline entry is for the headline force kernel, from
-HIP events recorded on the simulation's stream around each launch of a
-following window of up to 100 steps launched one by one; the measured VALU
-FMA rates (FP32 and packed FP16) and the VALU issue fraction from the
-committed rocprofv3 summary sit beside it. The CPU baseline times the FP64
-oracle's force pass (the reference CPU functor, src/force.h:31-60, over
-cell-list candidates, OpenMP) on the same developed state, on the host's
-cores. The
-serving entry runs a seeded Poisson stream of 16 dam-break requests through
-the continuous-batching loop (p50/p99 completion latency).
+runs the same steps beside it ("paths", "mixed_over_fp32"). The timed
+window starts at step 1,001 whatever --warmup is: the dam break is first
+run for 1,000 - warmup steps into its developed flow (--develop), then the
+warmup steps, then --steps timed steps (default: steps 1,001-2,000 of the
+configuration's 2,000). The roofline entry is for the headline force
+kernel, from HIP events recorded on the simulation's stream around each
+launch of a following window of up to 100 steps launched one by one; the
+measured VALU FMA rates (FP32 and packed FP16) and the VALU issue fraction
+from the committed rocprofv3 summary sit beside it. The CPU baseline times
+the FP64 oracle's force pass (the reference CPU functor, src/force.h:31-60,
+over cell-list candidates, OpenMP) on the same developed state, on the
+host's cores, built there with -O3 -march=native and again with the
+reference's -ffast-math -funroll-loops. The "fdps" entry times the FDPS
+drop-in (dispatch + retrieve, host records in and out) on the 3D state after
+200 steps. The serving entry runs a seeded Poisson stream of 16 dam-break
+requests through the continuous-batching loop, batched and one at a time
+(p50/p99 completion latency, aggregate pairs/s).
 """
 import argparse
 import json
@@ -40,21 +45,41 @@ PEAK_VALU_TFLOPS = 157.3             # MI355X published FP32 (packed) / packed-F
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline(sim, params, budget_s, threads):
-    import numpy as np
-    import oracle as O
-    out = sim.download()
-    st0 = sim.initial_state
-    s = O.State(sim.dim, sim.n)
-    s.x[:] = out["pos"]
-    s.v[:] = out["vel"]
-    s.dens[:] = out["dens"]
-    s.mass[:] = st0["mass"]
-    s.h[:] = st0["h"]
-    s.type[:] = st0["type"]
-    op = O.sim_params(sim.dim, alpha=params.alpha, beta=params.beta, eps=params.eps, c_cfl=params.c_cfl,
-                      vsig_mode=params.vsig_mode, rho0=params.rho0, c0=params.c0, gamma=params.gamma, g=params.g,
-                      nu=params.nu, rho_floor=params.rho_floor)
+def host_cpu():
+    """The host's CPU as the baseline record states it: logical CPUs (nproc),
+    the CPUs this process may run on, the model name and the physical core
+    count (distinct (package, core) pairs of /proc/cpuinfo)."""
+    info = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "model": None,
+            "physical_cores": None}
+    try:
+        cores, pkg, model = set(), None, None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model is None:
+                model = v
+            elif k == "physical id":
+                pkg = v
+            elif k == "core id":
+                cores.add((pkg, v))
+        info["model"] = model
+        info["physical_cores"] = len(cores) or None
+    except OSError:
+        pass
+    return info
+
+
+def oracle_force_rate(O, z, budget_s, threads):
+    dim, n = int(z["dim"]), len(z["dens"])
+    s = O.State(dim, n)
+    s.x[:] = z["pos"]
+    s.v[:] = z["vel"]
+    s.dens[:] = z["dens"]
+    s.mass[:] = z["mass"]
+    s.h[:] = z["h"]
+    s.type[:] = z["type"]
+    op = O.sim_params(dim, **{k: float(z[k]) for k in ("alpha", "beta", "eps", "c_cfl", "rho0", "c0", "gamma", "g",
+                                                        "nu", "rho_floor")}, vsig_mode=int(z["vsig_mode"]))
     s.eos(op)
     reps, pairs, t = 0, 0, 0.0
     while t < budget_s or reps == 0:
@@ -64,9 +89,59 @@ def cpu_baseline(sim, params, budget_s, threads):
         reps += 1
         if t > 0.5 * budget_s and t / reps * (reps + 1) > 1.5 * budget_s:
             break
-    return {"value": pairs / t, "unit": "pair-interactions/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} FP64 force pass(es) of the oracle (src/force.h:31-60 over 27-cell candidates) "
-                      f"on the developed {sim.n}-particle state, {t:.1f} s"}
+    return pairs / t, reps, t
+
+
+def cpu_baseline_worker(path, budget_s, threads):
+    """Child process of cpu_baseline: builds the oracle variants on this host
+    and times their force passes on the state saved at `path`; prints JSON."""
+    import tempfile
+    import numpy as np
+    import oracle as O
+    z = dict(np.load(path))
+    builds = {"O3_native": ["-O3", "-march=native"],
+              "O3_native_fast_math": ["-O3", "-march=native", "-ffast-math", "-funroll-loops"]}
+    rates = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, flags in builds.items():
+            O.use_library(O.build_variant(tmp, name, flags))
+            rate, reps, t = oracle_force_rate(O, z, budget_s / len(builds), threads)
+            rates[name] = {"value": rate, "passes": reps, "seconds": t, "flags": " ".join(flags + ["-fopenmp"])}
+    print(json.dumps(rates), flush=True)
+
+
+def cpu_baseline(sim, params, budget_s, threads):
+    """The oracle's FP64 force pass (src/force.h:31-60 over 27-cell
+    candidates, OpenMP) on the developed state, built on this host with
+    -O3 -march=native, and again with the reference's -ffast-math
+    -funroll-loops (src/Makefile:3); value is the first. It runs in a child
+    process with OMP_PROC_BIND=close and OMP_PLACES=cores: OpenMP binds the
+    thread that starts it, and threads this process creates afterwards
+    would inherit that binding."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    out = sim.download()
+    st0 = sim.initial_state
+    host = host_cpu()
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "state.npz")
+        np.savez(path, dim=sim.dim, pos=out["pos"], vel=out["vel"], dens=out["dens"], mass=st0["mass"],
+                 h=st0["h"], type=st0["type"], alpha=params.alpha, beta=params.beta, eps=params.eps,
+                 c_cfl=params.c_cfl, rho0=params.rho0, c0=params.c0, gamma=params.gamma, g=params.g, nu=params.nu,
+                 rho_floor=params.rho_floor, vsig_mode=params.vsig_mode)
+        env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(threads))
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-worker", path,
+                            "--cpu-seconds", str(budget_s), "--cpu-threads", str(threads)],
+                           env=env, check=True, capture_output=True, text=True)
+    rates = json.loads(r.stdout.strip().splitlines()[-1])
+    head = rates["O3_native"]
+    return {"value": head["value"], "unit": "pair-interactions/s", "cores": threads, "kind": "port",
+            "threads_used": threads, "omp_proc_bind": "close", "omp_places": "cores", **host, "builds": rates,
+            "sample": f"{head['passes']} FP64 force pass(es) of the oracle (src/force.h:31-60 over 27-cell "
+                      f"candidates) on the developed {sim.n}-particle state, {head['seconds']:.1f} s, built "
+                      f"-O3 -march=native -fopenmp on this host; the -ffast-math -funroll-loops build "
+                      f"(src/Makefile:3) is under builds"}
 
 
 def cpu_baseline_c0(threads, steps=1000):
@@ -362,6 +437,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--develop", type=int, default=-1,
+                    help="steps run before the warmup to bring the dam break into its developed flow "
+                         "(default: 1000 - warmup, so the timed window starts at step 1001)")
     ap.add_argument("--precision", default="mixed", choices=["fp32", "mixed", "fp64"],
                     help="headline path (the mixed and FP32 paths are both run unless --only)")
     ap.add_argument("--only", action="store_true", help="run the headline path alone")
@@ -384,7 +462,13 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-seed", type=int, default=2026)
+    ap.add_argument("--cpu-baseline-worker", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_worker:
+        cpu_baseline_worker(args.cpu_baseline_worker, args.cpu_seconds, args.cpu_threads or 1)
+        return
+    if args.develop < 0:
+        args.develop = max(0, 1000 - args.warmup)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -447,7 +531,8 @@ def main():
                                f"{args.precision} force path, full device-resident steps",
                    "particles": n, "dx": dx, "precision": args.precision,
                    "cells_per_support": args.cells_per_support or 2,
-                   "step_window": f"steps {args.warmup + 1}-{args.warmup + K}"},
+                   "develop_steps": args.develop,
+                   "step_window": f"steps {args.develop + args.warmup + 1}-{args.develop + args.warmup + K}"},
         "paths": runs,
         "roofline": {"bound": "valu", "kernel": kname, "achieved": achieved, "peak": PEAK_VALU_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_VALU_TFLOPS, "traffic": None,
@@ -473,17 +558,27 @@ def main():
     line["valu_probe_TFLOPs"] = peaks
     line["roofline"]["frac_of_measured"] = {k: achieved / v for k, v in peaks.items()}
     if rank == 0 and args.cpu_seconds > 0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the box's CPU share is 16 threads per GPU, whatever nproc shows
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)) or 1)
         print(f"# cpu baseline on {threads} threads", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(keep, params, args.cpu_seconds, threads)
         line["cpu_baseline_c0"] = cpu_baseline_c0(threads)
     keep.close()
+    if rank == 0 and dim == 3 and not args.only:
+        print("# FDPS drop-in", file=sys.stderr, flush=True)
+        fd = fdps_study(W, 3, 0.0126, 200)
+        line["fdps"] = {"workload": fd["workload"], "end_to_end_ms": fd["end_to_end_ms"],
+                        "kernel_ms": fd["kernel_ms"], "pack_ms": fd["stages_ms"]["pack_ms"],
+                        "unpack_ms": fd["stages_ms"]["unpack_ms"],
+                        "pairs_per_s_end_to_end": fd["pairs_per_s_end_to_end"],
+                        "pairs_per_s_kernel": fd["pairs_per_s_kernel"]}
     if rank == 0 and args.serve_requests > 0 and not args.only:
         print(f"# serving {args.serve_requests} requests", file=sys.stderr, flush=True)
         sv = serving_study(W, args.serve_seed, args.serve_requests, args.serve_rate, args.serve_slots, 50,
-                           "mixed", 64.0, compare=False)
-        line["serving"] = {"workload": sv["workload"], "max_slots": sv["max_slots"],
-                           **sv["runs"][f"slots_{args.serve_slots}"]}
+                           "mixed", 64.0, compare=True)
+        line["serving"] = {"workload": sv["workload"], "max_slots": sv["max_slots"], "runs": sv["runs"],
+                           "results_identical_batched_vs_one_at_a_time":
+                               sv["results_identical_batched_vs_one_at_a_time"]}
     if rank == 0:
         print(json.dumps(line), flush=True)
 
@@ -500,9 +595,10 @@ def run_path(W, params, st, prec, args):
     sim.initial_state = st
     sim.init()
     t0 = time.perf_counter()
-    sim.step(args.warmup)
+    sim.step(args.develop + args.warmup)
     sim.sync()
-    print(f"# {prec}: warmup {args.warmup} steps {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+    print(f"# {prec}: {args.develop} developing + {args.warmup} warmup steps {time.perf_counter() - t0:.2f} s",
+          file=sys.stderr, flush=True)
     s0 = sim.stats()
     t0 = time.perf_counter()
     sim.step(args.steps)

@@ -50,53 +50,80 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def build_variant(outdir, name, flags):
+    """Compile the oracle with other optimisation flags (the CPU baseline's
+    -O3 -march=native builds, made on the machine that runs them) into
+    outdir/liboracle_<name>.so; returns its path."""
+    out = os.path.join(outdir, f"liboracle_{name}.so")
+    subprocess.run(["gcc", *flags, "-fPIC", "-fopenmp", "-std=c99", "-shared", "-o", out,
+                    os.path.join(_HERE, "wcsph_oracle.c"), "-lm"], check=True)
+    return out
+
+
+def use_library(path=None):
+    """Route the oracle's functions to the build at `path` (None: the
+    default liboracle.so). Returns the path that was in use before."""
+    global _lib, _lib_path
+    prev = _lib_path
+    _lib = _load(path or _LIB)
+    _lib_path = path
+    return prev
+
+
+_lib_path = None
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
-        L = C.CDLL(_LIB)
-        L.orc_kernel_W.restype = C.c_double
-        L.orc_kernel_W.argtypes = [C.c_int, C.c_double, C.c_double]
-        L.orc_kernel_gradW_factor.restype = C.c_double
-        L.orc_kernel_gradW_factor.argtypes = [C.c_int, C.c_double, C.c_double]
-        L.orc_eos_pressure.restype = C.c_double
-        L.orc_eos_pressure.argtypes = [C.c_double] * 4
-        L.orc_eos_soundspeed.restype = C.c_double
-        L.orc_eos_soundspeed.argtypes = [C.c_double] * 4
-        L.orc_hydro_force.argtypes = [C.POINTER(ForceParams), f64p, C.c_int, f64p, C.c_int, f64p]
-        L.orc_result_clear.argtypes = [C.c_int, f64p, C.c_int]
-        L.orc_hydro_force_walks.argtypes = [C.POINTER(ForceParams), C.c_int, f64p, i64p, f64p, i64p,
-                                            f64p, C.c_int]
-        L.orc_nbr_pred.restype = C.c_int
-        L.orc_neighbours_brute.restype = C.c_int
-        L.orc_neighbours_brute.argtypes = [C.c_int, C.c_int64, f64p, f32p, i32p, C.c_void_p, C.c_int]
-        L.orc_neighbours_brute_cells.restype = C.c_int
-        L.orc_neighbours_brute_cells.argtypes = [C.c_int, C.c_int64, f64p, f32p, f64p, C.c_double, i32p, i32p,
-                                                 C.c_void_p, C.c_int]
-        L.orc_neighbour_counts_cells.restype = C.c_int64
-        L.orc_neighbour_counts_cells.argtypes = [C.c_int, C.c_int64, f64p, f32p, f64p, C.c_double, i32p, i32p,
-                                                 C.c_int]
-        L.orc_make_walks.restype = C.c_int64
-        L.orc_make_walks.argtypes = [C.c_int, C.c_int64, f64p, C.c_double, C.c_int,
-                                     C.POINTER(C.c_int64), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.orc_shepard.argtypes = [C.c_int, C.c_int64, f64p, f64p, f64p, f64p, f64p, C.c_int]
-        L.orc_ic_dam_break.restype = C.c_int64
-        L.orc_ic_dam_break.argtypes = [C.c_int] + [C.c_double] * 6 + [C.c_int] + [C.c_double] * 5 + \
-                                      [C.POINTER(_CState)]
-        L.orc_sim_init.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), C.c_int]
-        L.orc_sim_step.restype = C.c_int64
-        L.orc_sim_step.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), C.c_int]
-        L.orc_sim_eos.argtypes = [C.POINTER(SimParams), C.POINTER(_CState)]
-        L.orc_sim_force.restype = C.c_int64
-        L.orc_sim_force.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), C.c_int]
-        L.orc_global_dt.restype = C.c_double
-        L.orc_global_dt.argtypes = [C.POINTER(SimParams), C.POINTER(_CState)]
-        L.orc_conservation.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), f64p]
-        L.orc_dam_front.restype = C.c_double
-        L.orc_dam_front.argtypes = [C.POINTER(_CState), C.c_int, C.c_double]
-        _lib = L
+        _lib = _load(_LIB)
     return _lib
+
+
+def _load(path):
+    L = C.CDLL(path)
+    L.orc_kernel_W.restype = C.c_double
+    L.orc_kernel_W.argtypes = [C.c_int, C.c_double, C.c_double]
+    L.orc_kernel_gradW_factor.restype = C.c_double
+    L.orc_kernel_gradW_factor.argtypes = [C.c_int, C.c_double, C.c_double]
+    L.orc_eos_pressure.restype = C.c_double
+    L.orc_eos_pressure.argtypes = [C.c_double] * 4
+    L.orc_eos_soundspeed.restype = C.c_double
+    L.orc_eos_soundspeed.argtypes = [C.c_double] * 4
+    L.orc_hydro_force.argtypes = [C.POINTER(ForceParams), f64p, C.c_int, f64p, C.c_int, f64p]
+    L.orc_result_clear.argtypes = [C.c_int, f64p, C.c_int]
+    L.orc_hydro_force_walks.argtypes = [C.POINTER(ForceParams), C.c_int, f64p, i64p, f64p, i64p,
+                                        f64p, C.c_int]
+    L.orc_nbr_pred.restype = C.c_int
+    L.orc_neighbours_brute.restype = C.c_int
+    L.orc_neighbours_brute.argtypes = [C.c_int, C.c_int64, f64p, f32p, i32p, C.c_void_p, C.c_int]
+    L.orc_neighbours_brute_cells.restype = C.c_int
+    L.orc_neighbours_brute_cells.argtypes = [C.c_int, C.c_int64, f64p, f32p, f64p, C.c_double, i32p, i32p,
+                                             C.c_void_p, C.c_int]
+    L.orc_neighbour_counts_cells.restype = C.c_int64
+    L.orc_neighbour_counts_cells.argtypes = [C.c_int, C.c_int64, f64p, f32p, f64p, C.c_double, i32p, i32p,
+                                             C.c_int]
+    L.orc_make_walks.restype = C.c_int64
+    L.orc_make_walks.argtypes = [C.c_int, C.c_int64, f64p, C.c_double, C.c_int,
+                                 C.POINTER(C.c_int64), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_shepard.argtypes = [C.c_int, C.c_int64, f64p, f64p, f64p, f64p, f64p, C.c_int]
+    L.orc_ic_dam_break.restype = C.c_int64
+    L.orc_ic_dam_break.argtypes = [C.c_int] + [C.c_double] * 6 + [C.c_int] + [C.c_double] * 5 + \
+                                  [C.POINTER(_CState)]
+    L.orc_sim_init.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), C.c_int]
+    L.orc_sim_step.restype = C.c_int64
+    L.orc_sim_step.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), C.c_int]
+    L.orc_sim_eos.argtypes = [C.POINTER(SimParams), C.POINTER(_CState)]
+    L.orc_sim_force.restype = C.c_int64
+    L.orc_sim_force.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), C.c_int]
+    L.orc_global_dt.restype = C.c_double
+    L.orc_global_dt.argtypes = [C.POINTER(SimParams), C.POINTER(_CState)]
+    L.orc_conservation.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), f64p]
+    L.orc_dam_front.restype = C.c_double
+    L.orc_dam_front.argtypes = [C.POINTER(_CState), C.c_int, C.c_double]
+    return L
 
 
 # ----------------------------------------------------------------------------
