@@ -27,8 +27,10 @@ host's cores, built there with -O3 -march=native and again with the
 reference's -ffast-math -funroll-loops. The "fdps" entry times the FDPS
 drop-in (dispatch + retrieve, host records in and out) on the 3D state after
 200 steps. The serving entry runs a seeded Poisson stream of 16 dam-break
-requests through the continuous-batching loop, batched and one at a time
-(p50/p99 completion latency, aggregate pairs/s).
+requests at 8/s through the continuous-batching loop (grouped step kernels,
+16 slots), batched and one at a time (p50/p99 completion latency, aggregate
+pairs/s); "serving_sustained" runs 48 requests at 3/s, near the rate the
+mix sustains.
 """
 import argparse
 import json
@@ -459,8 +461,12 @@ def main():
     ap.add_argument("--serve", action="store_true",
                     help="run a seeded request stream through the serving loop (p50/p99 latency, makespan)")
     ap.add_argument("--serve-requests", type=int, default=16, help="requests of the serving run (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate, requests/s")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--sustained-requests", type=int, default=48,
+                    help="requests of the default run's second serving stream, at --sustained-rate (0 skips it)")
+    ap.add_argument("--sustained-rate", type=float, default=3.0,
+                    help="arrival rate near what the mix sustains, requests/s")
     ap.add_argument("--serve-seed", type=int, default=2026)
     ap.add_argument("--cpu-baseline-worker", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -579,6 +585,13 @@ def main():
         line["serving"] = {"workload": sv["workload"], "max_slots": sv["max_slots"], "runs": sv["runs"],
                            "results_identical_batched_vs_one_at_a_time":
                                sv["results_identical_batched_vs_one_at_a_time"]}
+        if args.sustained_requests > 0:
+            print(f"# serving {args.sustained_requests} requests at {args.sustained_rate}/s", file=sys.stderr,
+                  flush=True)
+            su = serving_study(W, args.serve_seed, args.sustained_requests, args.sustained_rate, args.serve_slots,
+                               50, "mixed", 64.0, compare=False)
+            line["serving_sustained"] = {"workload": su["workload"], "max_slots": su["max_slots"],
+                                         **su["runs"][f"slots_{args.serve_slots}"]}
     if rank == 0:
         print(json.dumps(line), flush=True)
 

@@ -174,9 +174,18 @@ int wcsph_dam_break_count(const wcsph_dam_break_desc* d, int64_t* n_fluid, int64
  * leaves commented out at src/init/Dam2.h:73-74). */
 int wcsph_dam_break_fill(const wcsph_dam_break_desc* d, const wcsph_params* p, double* pos,
                          double* vel, double* dens, double* mass, double* h, int32_t* type);
-/* Device bytes a simulation of n particles needs (for admission control).
- * No counterpart in the reference. */
+/* Device bytes a simulation of n particles needs at cfg's precision
+ * (neighbour lists only for FP64), assuming a cell grid of about one cell
+ * per particle; the grid an upload builds depends on the state, so this is
+ * an estimate. No counterpart in the reference. */
 int64_t wcsph_sim_bytes(int32_t dim, int64_t n, const wcsph_sim_config* cfg);
+/* Exact device bytes of a simulation created with p and cfg and uploaded
+ * with this state (pos [dim][n], h [n]): the grid is computed on the host as
+ * the upload would build it. -1 on a null or inconsistent argument. The
+ * serving loop admits requests by this figure. No counterpart in the
+ * reference. */
+int64_t wcsph_sim_bytes_state(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t n, const double* pos,
+                              const double* h);
 
 /* Create / destroy the device state of one simulation: the role of FDPS's
  * ParticleSystem and tree set-up in src/main.cpp:33-67, held in HBM. */
@@ -282,11 +291,18 @@ typedef struct {
 } wcsph_serve_config;
 
 /* Run the whole request stream: admit arrived requests (FIFO) while a slot
- * is free and the budget allows, step each admitted request on its own
- * stream from a captured two-step graph, retire finished ones and record
- * per-request timing (latency = end_s - arrival_s) in out[] (same order as
- * reqs). Returns when every request is done. No counterpart in the
- * reference, which runs one simulation per process. */
+ * is free and the budget allows (by each request's exact footprint,
+ * wcsph_sim_bytes_state), retire finished ones and record per-request
+ * timing (latency = end_s - arrival_s) in out[] (same order as reqs).
+ * Device memory is one arena of hbm_budget_bytes allocated at entry;
+ * requests take ranges of it, so nothing is allocated or freed on the
+ * device while requests run. Mixed-precision requests without the Shepard
+ * filter share grouped step kernels (one launch per stage of the step for
+ * all such requests of one dimension and pair-term switches, replayed from
+ * a two-step graph re-captured when the group changes); others step on a
+ * stream of their own. max_slots is at most 16. Returns when every request
+ * is done. No counterpart in the reference, which runs one simulation per
+ * process. */
 int wcsph_serve(const wcsph_serve_config* cfg, const wcsph_request* reqs, int32_t n, wcsph_request_result* out);
 
 /* ---------------- 4. snapshots and restart ---------------- */

@@ -4,8 +4,10 @@
   eagerly launched steps bit for bit, replay after replay, with downloads
   and synchronisations in between;
 * serving: every request of a mixed stream gives bit-identical results to
-  running it alone (no shared device state, deterministic kernels); FIFO
-  admission respects the slot limit and the HBM budget.
+  running it alone (its grouped kernels touch only its own buffers, every
+  kernel is deterministic), and the grouped step kernels give the ordinary
+  kernels' results; FIFO admission respects the slot limit and the HBM
+  budget, admitted by exact footprints from one arena.
 """
 import numpy as np
 import pytest
@@ -67,16 +69,10 @@ def test_serving_fifo_one_slot(W):
 
 
 def test_serving_budget_serialises(W):
-    import ctypes as C
     reqs = small_stream(W, precision="fp32")
-    # a budget that holds any one request but never the two largest at once
-    sizes = []
-    for r in reqs:
-        nf, nw = C.c_int64(), C.c_int64()
-        W.check(W._abi.lib().wcsph_dam_break_count(C.byref(r.geom), C.byref(nf), C.byref(nw)))
-        cfg = W.SimConfig()
-        cfg.precision = r.precision
-        sizes.append(W._abi.lib().wcsph_sim_bytes(r.geom.dim, nf.value + nw.value, C.byref(cfg)))
+    # a budget that holds any one request but never the two largest at once,
+    # by the exact footprints the loop admits by
+    sizes = [W.request_bytes(r) for r in reqs]
     budget = max(sizes) + min(sizes) - 1
     res = W.serve(reqs, max_slots=4, hbm_budget_bytes=budget, chunk_steps=50)
     ev = sorted([(r["start_s"], 1, sizes[k]) for k, r in enumerate(res)] + [(r["end_s"], -1, sizes[k]) for k, r in
@@ -86,6 +82,39 @@ def test_serving_budget_serialises(W):
         used += sign * sz
         assert used <= budget
     assert all(r["status"] == 0 for r in res)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp32", "fp64"])
+def test_serving_footprint_is_exact(W, precision):
+    # an arena of exactly one request's footprint holds all its buffers (the
+    # figure is not an underestimate), and one byte less is refused up front
+    r = W.make_request(2, 20_000, 1.0, 0.05, 20, 0.0, precision)
+    b = W.request_bytes(r)
+    res = W.serve([r], max_slots=1, hbm_budget_bytes=b, chunk_steps=10)[0]
+    assert res["status"] == 0 and res["steps"] == 20
+    with pytest.raises(W.WcsphError, match="budget"):
+        W.serve([r], max_slots=1, hbm_budget_bytes=b - 1)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_grouped_steps_equal_simulation(W, dim):
+    # the grouped step kernels (serving) give the results of the ordinary
+    # step kernels (a Simulation stepped directly), bit for bit, also when
+    # the step count is odd (the last step runs as a one-member launch)
+    r = W.make_request(dim, 30_000, 1.5, 0.1, 121, 0.0, "mixed")
+    res = W.serve([r], max_slots=1, chunk_steps=40)[0]
+    assert res["status"] == 0 and res["precision_used"] == W.PREC_MIXED
+    st = W.dam_break_state(r.geom, r.params)
+    band = 2 * r.geom.dx
+    with W.Simulation.from_state(r.params, st, precision="mixed") as sim:
+        sim.init()
+        sim.step(121)
+        s = sim.stats()
+        d = sim.diagnostics(band, band, r.geom.tank_l - band)
+    assert res["steps"] == s["step"] == 121
+    assert res["pairs"] == s["pairs_total"]
+    assert res["sim_time"] == s["time"]
+    assert res["front"] == d["front"] and res["column"] == d["column"] and res["kinetic"] == d["kinetic"]
 
 
 def test_serving_rejects_oversized_request(W):

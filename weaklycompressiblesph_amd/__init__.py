@@ -450,6 +450,19 @@ def poisson_workload(seed, count, rate, *, n_range=(20_000, 2_000_000), aspect=(
     return reqs
 
 
+def request_bytes(r, precision=None):
+    """Exact device bytes of request r as the serving loop admits it (its
+    initial state's grid computed on the host), at its own precision or the
+    one named."""
+    st = dam_break_state(r.geom, r.params)
+    cfg = SimConfig()
+    cfg.precision = r.precision if precision is None else PRECISIONS[precision]
+    b = lib().wcsph_sim_bytes_state(C.byref(r.params), C.byref(cfg), len(st["dens"]), ptr(st["pos"]), ptr(st["h"]))
+    if b < 0:
+        raise WcsphError(1, "wcsph_sim_bytes_state rejected the request")
+    return b
+
+
 def serve(requests, *, hbm_budget_bytes=64 << 30, max_slots=4, chunk_steps=50):
     """Run a request stream through the continuous-batching loop. Returns one
     dict per request (same order): arrival/start/end wall times, latency,

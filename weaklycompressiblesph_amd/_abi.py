@@ -99,6 +99,7 @@ EXPORTS = [
     "wcsph_init", "wcsph_finalize",
     "wcsph_hydro_dispatch_3d", "wcsph_hydro_dispatch_2d", "wcsph_hydro_retrieve_3d", "wcsph_hydro_retrieve_2d",
     "wcsph_hydro_last_kernel_ms", "wcsph_hydro_last_timing", "wcsph_dam_break_count", "wcsph_dam_break_fill", "wcsph_sim_bytes",
+    "wcsph_sim_bytes_state",
     "wcsph_sim_create", "wcsph_sim_destroy", "wcsph_sim_upload", "wcsph_sim_init", "wcsph_sim_step",
     "wcsph_sim_force", "wcsph_sim_sync", "wcsph_sim_get_stats", "wcsph_sim_download", "wcsph_sim_neighbours",
     "wcsph_sim_diagnostics", "wcsph_sim_last_force_ms", "wcsph_sim_set_timing", "wcsph_sim_timing",
@@ -139,6 +140,7 @@ def lib():
         "wcsph_dam_break_count": (C.c_int, [C.POINTER(DamBreakDesc), C.POINTER(I64), C.POINTER(I64)]),
         "wcsph_dam_break_fill": (C.c_int, [C.POINTER(DamBreakDesc), C.POINTER(Params), VP, VP, VP, VP, VP, VP]),
         "wcsph_sim_bytes": (I64, [I32, I64, C.POINTER(SimConfig)]),
+        "wcsph_sim_bytes_state": (I64, [C.POINTER(Params), C.POINTER(SimConfig), I64, C.c_void_p, C.c_void_p]),
         "wcsph_sim_create": (C.c_int, [C.POINTER(Params), C.POINTER(SimConfig), I64, C.POINTER(VP)]),
         "wcsph_sim_destroy": (C.c_int, [VP]),
         "wcsph_sim_upload": (C.c_int, [VP, VP, VP, VP, VP, VP, VP]),

@@ -219,8 +219,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, int
 }
 
 template <int D, int R, int VS, int BT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WCSPH_MIX_WAVES)))
-void k_nforce_mixed(NFMixedArgs a) {
+__device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& a) {
     constexpr int S = 2 * R + 1;
     constexpr int NR = D == 3 ? S * S : S;
     constexpr int Q = D == 3 ? kMixQ : kMixQ2, K = D == 3 ? kMixK : kMixK2, U = kMixU;
@@ -249,7 +248,7 @@ void k_nforce_mixed(NFMixedArgs a) {
     // they pop), so a drain evaluates slots above the top with no select
 #pragma unroll
     for (int s = 0; s < Q; ++s) q[s * 64] = 0u;
-    const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)xcd_block(vb) * blockDim.x + threadIdx.x;
     const bool failed = sim_failed(a.s);
     const bool live = i < a.n && !failed;
     const int64_t ii = live ? i : 0;
@@ -483,7 +482,13 @@ void k_nforce_mixed(NFMixedArgs a) {
         for (int d = 0; d < D; ++d) a.acc[d * n + i] = ad[d];
         --total;  // the particle itself passed the predicate (r = 0) and added zero
     }
-    block_count_atomic(total, a.s, true, a.fused_final ? &a.P.dtmin[blockIdx.x] : nullptr, cand);
+    block_count_atomic(total, a.s, true, a.fused_final ? &a.P.dtmin[vb.id] : nullptr, cand);
+}
+
+template <int D, int R, int VS, int BT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WCSPH_MIX_WAVES)))
+void k_nforce_mixed(NFMixedArgs a) {
+    nforce_mixed_body<D, R, VS, BT>(hw_block(), a);
 }
 
 }  // namespace wcsph

@@ -23,77 +23,79 @@
 #include <cstring>
 
 #include "common.hpp"
-#include "mixed_kernels.hpp"
-#include "sim_kernels.hpp"
+#include "group_kernels.hpp"
+#include "sim.hpp"
 
 using namespace wcsph;
 
-struct wcsph_sim {
-    wcsph_params p{};
-    wcsph_sim_config cfg{};
-    int D = 3;
-    int64_t n = 0;
-    int K = 128;
-    int device = 0;
-    hipStream_t st = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    Grid grid{};
-    int64_t ncell = 0, npart = 0;
-    Master m[2]{};
-    int cur = 0;
-    double *acc = nullptr, *divv = nullptr, *dti = nullptr;
-    Packed pk{};
-    int R = 2;      // cells per support length
-    int Q = 16;     // LDS queue depth of the fused pass
-    float hmax = 0.f;
-    bool uniform_h = false;  // all smoothing lengths equal in FP32 (fixed in time)
-    double h_const = 0.0;    // > 0: all smoothing lengths are exactly this (kept in both buffers, not permuted)
-    double *pr64 = nullptr, *c64 = nullptr;  // FP64 path only
-    Mixed mx{};                               // mixed path only
-    MixedConsts mc{};
-    uint32_t *key = nullptr, *slot = nullptr, *count = nullptr, *start = nullptr, *perm = nullptr;
-    uint32_t *tile_total = nullptr, *group_total = nullptr;  // scan tile and tile-group totals
-    int64_t ngroups = 0;
-    uint32_t *nbr = nullptr, *cnt = nullptr;
-    Scalars* sc = nullptr;
-    Partials P{};
-    double* diag_part = nullptr;
-    double* diag_out = nullptr;
-    bool uploaded = false, forced = false;
-    bool steppable = false;  // acc, the dt bounds and the clock are those of init or a step
-    bool fuse_final = false;  // the next fused force pass also runs the final stage of a step
-    bool timing = false;
-    bool use_graphs = true;
-    int64_t hstep = 0;         // steps queued since init (the Shepard filter schedule)
-    double* rho_tmp = nullptr;  // filtered densities before they are committed
-    hipGraphExec_t graph2[2] = {nullptr, nullptr};  // two steps starting from cur = 0 / 1
-    std::vector<hipEvent_t> tev;  // 5 events per timed step, pooled
-    int64_t tsteps = 0;
-    std::vector<void*> allocs;
-    // pinned copy of the device failure flags (non-finite positions, overfull
-    // cells), refreshed after every call that queues steps
-    volatile uint32_t* hflags = nullptr;
+void* wcsph_arena::take(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    for (size_t k = 0; k < free_list.size(); ++k) {
+        auto& f = free_list[k];
+        if (f.second < bytes) continue;
+        void* p = base + f.first;
+        f.first += bytes;
+        f.second -= bytes;
+        if (f.second == 0) free_list.erase(free_list.begin() + (std::ptrdiff_t)k);
+        in_use += bytes;
+        return p;
+    }
+    return nullptr;
+}
 
-    template <class T>
-    T* alloc(int64_t count) {
-        void* ptr = nullptr;
-        WCSPH_HIP(hipMalloc(&ptr, std::max<int64_t>(count, 1) * sizeof(T)));
-        allocs.push_back(ptr);
-        return (T*)ptr;
+void wcsph_arena::give(void* p, size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    const size_t off = (size_t)((char*)p - base);
+    auto it = std::lower_bound(free_list.begin(), free_list.end(), std::make_pair(off, (size_t)0));
+    it = free_list.insert(it, {off, bytes});
+    in_use -= bytes;
+    // merge with the following, then the preceding range
+    auto next = it + 1;
+    if (next != free_list.end() && it->first + it->second == next->first) {
+        it->second += next->second;
+        free_list.erase(next);
     }
-    ~wcsph_sim() {
-        if (st) (void)hipStreamSynchronize(st);
-        for (void* a : allocs) (void)hipFree(a);
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        for (auto& e : tev) (void)hipEventDestroy(e);
-        for (auto& g : graph2)
-            if (g) (void)hipGraphExecDestroy(g);
-        if (st) (void)hipStreamDestroy(st);
-        if (hflags) (void)hipHostFree((void*)hflags);
+    if (it != free_list.begin()) {
+        auto prev = it - 1;
+        if (prev->first + prev->second == it->first) {
+            prev->second += it->second;
+            free_list.erase(it);
+        }
     }
-    int64_t nb(int64_t items) const { return ceil_div(items, kBlock); }
-};
+}
+
+size_t wcsph_arena::largest_free() const {
+    size_t m = 0;
+    for (const auto& f : free_list) m = std::max(m, f.second);
+    return m;
+}
+
+void* wcsph_sim::alloc_bytes(size_t bytes) {
+    void* ptr = nullptr;
+    if (arena) {
+        ptr = arena->take(bytes);
+        WCSPH_REQUIRE(ptr, WCSPH_ENOMEM, "the serving arena has no free range for a buffer of this request");
+    } else {
+        WCSPH_HIP(hipMalloc(&ptr, bytes));
+    }
+    allocs.push_back({ptr, bytes});
+    return ptr;
+}
+
+wcsph_sim::~wcsph_sim() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (auto& a : allocs) {
+        if (arena) arena->give(a.first, a.second);
+        else (void)hipFree(a.first);
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : tev) (void)hipEventDestroy(e);
+    for (auto& g : graph2)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (st && own_stream) (void)hipStreamDestroy(st);
+    if (hflags && own_flags) (void)hipHostFree((void*)hflags);
+}
 
 namespace {
 
@@ -105,13 +107,117 @@ void check_params(const wcsph_params* p) {
     WCSPH_REQUIRE(p->shepard_interval >= 0, WCSPH_EINVAL, "shepard_interval must be non-negative");
 }
 
-int64_t sim_bytes(int D, int64_t n, int K, int64_t ncell_guess) {
-    const int64_t master = (3 * D * 8 + 3 * 8 + 1 + 4) * n;  // pos vel vh, dens mass h, type, id
-    const int64_t per = 2 * master + (D * 8 + 16) * n + 52 * n + 4 * 3 * n + 8 * n + (int64_t)K * 4 * n + 4 * n;
-    return per + 8 * (ncell_guess + 1) + (1 << 20);
+constexpr int kDiag = 10;  // diagnostic sums
+
+// The cell grid of a state: cells slightly wider than the largest support
+// 2 h_max / R, over the configured box or the particles' bounding box with
+// two cells of margin and half the height again above (room for splashes).
+struct GridPlan {
+    double lo[3] = {0, 0, 0};
+    double L = 0;
+    int32_t nc[3] = {1, 1, 1};
+    int64_t ncell = 1;
+};
+
+GridPlan plan_grid(int D, int R, const wcsph_sim_config& cfg, int64_t n, const double* pos, float hmax) {
+    GridPlan G;
+    // cell edge slightly above the largest support 2 h_max, so any pair that
+    // passes the FP32 predicate lies in adjacent cells
+    G.L = 2.0 * (double)hmax * (1.0 + 1.0e-5) / R;
+    double hi[3] = {0, 0, 0};
+    bool have_box = true;
+    for (int d = 0; d < D; ++d) have_box = have_box && cfg.box_hi[d] > cfg.box_lo[d];
+    for (int d = 0; d < D; ++d) {
+        if (have_box) {
+            G.lo[d] = cfg.box_lo[d];
+            hi[d] = cfg.box_hi[d];
+        } else {
+            G.lo[d] = hi[d] = pos[d * n];
+            for (int64_t i = 1; i < n; ++i) {
+                G.lo[d] = std::min(G.lo[d], pos[d * n + i]);
+                hi[d] = std::max(hi[d], pos[d * n + i]);
+            }
+            const double ext = hi[d] - G.lo[d];
+            G.lo[d] -= 2.0 * G.L;
+            hi[d] += 2.0 * G.L + (d == D - 1 ? 0.5 * ext : 0.0);  // room above for splashes
+        }
+    }
+    G.ncell = 1;
+    for (int d = 0; d < D; ++d) {
+        G.nc[d] = (int32_t)std::max<int64_t>(1, (int64_t)std::ceil((hi[d] - G.lo[d]) / G.L));
+        G.ncell *= G.nc[d];
+    }
+    return G;
+}
+
+float max_h_fp32(int64_t n, const double* h) {
+    double hmax = 0.0;
+    for (int64_t i = 0; i < n; ++i) hmax = std::max(hmax, (double)(float)h[i]);
+    return (float)hmax;
 }
 
 }  // namespace
+
+namespace wcsph {
+
+int64_t sim_footprint(int D, int64_t n, int precision, int K, int64_t ncell) {
+    int64_t total = 0;
+    auto add = [&total](int64_t bytes) { total += (std::max<int64_t>(bytes, 1) + 255) & ~(int64_t)255; };
+    const int64_t nb = ceil_div(n, kBlock);
+    for (int b = 0; b < 2; ++b) {  // master state, double-buffered
+        for (int k = 0; k < 3; ++k) add(8 * D * n);  // pos vel vh
+        for (int k = 0; k < 3; ++k) add(8 * n);      // dens mass h
+        add(n);                                      // type
+        add(4 * n);                                  // id
+    }
+    add(8 * D * n);                          // acc
+    for (int k = 0; k < 3; ++k) add(8 * n);  // rho_tmp divv dti
+    for (int k = 0; k < 3; ++k) add(16 * n);  // FP32 records px pv pq
+    add(4 * n);                               // pcell
+    if (precision == WCSPH_PREC_FP64) {
+        add(8 * n);  // pr64
+        add(8 * n);  // c64
+    }
+    if (precision == WCSPH_PREC_MIXED) {
+        const int64_t r = mixed_records(n);
+        add(r * (int64_t)sizeof(PosRec16));
+        add(r * (int64_t)sizeof(VelRec16));
+        add(r * (int64_t)sizeof(StateRec16));
+        add(r * (int64_t)sizeof(h2));
+    }
+    for (int k = 0; k < 3; ++k) add(4 * n);  // key slot perm
+    add(sizeof(Scalars));
+    add(8 * nb);  // pairs
+    add(4 * nb);  // maxcnt
+    add(4 * nb);  // over
+    add(8 * nb);  // dtmin
+    add(8 * nb * kDiag);
+    add(8 * kDiag);
+    // cell grid: counts and starts with the dead bucket, scan totals
+    const int64_t npart = ceil_div(ncell + 1, kScanTile);
+    add(4 * (ncell + 1));
+    add(4 * (ncell + 2));
+    add(4 * npart);
+    add(4 * ceil_div(npart, scan_group_size(npart)));
+    if (precision == WCSPH_PREC_FP64) {  // neighbour lists of the FP64 pass
+        add(4 * (int64_t)K * n);
+        add(4 * n);
+    }
+    return total;
+}
+
+int64_t sim_grid_cells(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t n, const double* pos,
+                       const double* h, int32_t* nc) {
+    wcsph_sim_config c{};
+    if (cfg) c = *cfg;
+    const int R = c.cells_per_support == 0 ? 2 : c.cells_per_support;
+    const GridPlan G = plan_grid(p->dim, R, c, n, pos, max_h_fp32(n, h));
+    if (nc)
+        for (int d = 0; d < 3; ++d) nc[d] = G.nc[d];
+    return G.ncell;
+}
+
+}  // namespace wcsph
 
 // Copy the device failure flags to the pinned mirror behind the queued work.
 static void queue_flags(wcsph_sim* s) {
@@ -135,8 +241,7 @@ static void raise_if_failed(const wcsph_sim* s) {
 // whether all are equal.
 static void scan_h(wcsph_sim* s, const double* h) {
     const int64_t n = s->n;
-    double hmax = 0.0;
-    for (int64_t i = 0; i < n; ++i) hmax = std::max(hmax, (double)(float)h[i]);
+    const double hmax = max_h_fp32(n, h);
     s->uniform_h = true;
     bool same = true;
     for (int64_t i = 0; i < n; ++i) {
@@ -174,33 +279,9 @@ static void alloc_grid(wcsph_sim* s, const double* lo, double L, const int32_t* 
 }
 
 static void build_grid(wcsph_sim* s, const double* pos, const double* h) {
-    const int D = s->D;
-    const int64_t n = s->n;
     scan_h(s, h);
-    // cell edge slightly above the largest support 2 h_max, so any pair that
-    // passes the FP32 predicate lies in adjacent cells
-    const double L = 2.0 * (double)s->hmax * (1.0 + 1.0e-5) / s->R;
-    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    bool have_box = true;
-    for (int d = 0; d < D; ++d) have_box = have_box && s->cfg.box_hi[d] > s->cfg.box_lo[d];
-    for (int d = 0; d < D; ++d) {
-        if (have_box) {
-            lo[d] = s->cfg.box_lo[d];
-            hi[d] = s->cfg.box_hi[d];
-        } else {
-            lo[d] = hi[d] = pos[d * n];
-            for (int64_t i = 1; i < n; ++i) {
-                lo[d] = std::min(lo[d], pos[d * n + i]);
-                hi[d] = std::max(hi[d], pos[d * n + i]);
-            }
-            const double ext = hi[d] - lo[d];
-            lo[d] -= 2.0 * L;
-            hi[d] += 2.0 * L + (d == D - 1 ? 0.5 * ext : 0.0);  // room above for splashes
-        }
-    }
-    int32_t nc[3] = {1, 1, 1};
-    for (int d = 0; d < D; ++d) nc[d] = (int32_t)std::max<int64_t>(1, (int64_t)std::ceil((hi[d] - lo[d]) / L));
-    alloc_grid(s, lo, L, nc);
+    const GridPlan G = plan_grid(s->D, s->R, s->cfg, s->n, pos, s->hmax);
+    alloc_grid(s, G.lo, G.L, G.nc);
 }
 
 // Checks and constants of the precision path that depend on the grid and h.
@@ -222,52 +303,81 @@ static void setup_precision(wcsph_sim* s) {
     s->mc.half_Ls = (float)(0.5 * s->grid.L / hh);
 }
 
+// Argument records of the step kernels for the current state.
+static ScalarsArgs scalars_args(const wcsph_sim* s, int mode) { return ScalarsArgs{s->sc, s->P.dtmin, s->P.nblk, mode}; }
+
+template <int D>
+static BinArgs<D> bin_args(const wcsph_sim* s, int kick) {
+    return BinArgs<D>{s->m[s->cur], s->acc, s->divv, kick, s->n, s->grid, s->key, s->slot, s->count, s->sc};
+}
+
+static ScanArgs scan_args(const wcsph_sim* s) {
+    return ScanArgs{s->count, s->ncell + 1, s->tile_total, s->group_total, scan_group_size(s->npart), s->start};
+}
+
+static ScatterArgs scatter_args(const wcsph_sim* s) {
+    return ScatterArgs{s->key, s->slot, s->start, s->perm, s->count, s->n};
+}
+
+template <int D, class Extra>
+static ReorderArgs<D, Extra> reorder_args(const wcsph_sim* s, int kick, Extra extra) {
+    const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
+    const bool mixed = s->cfg.precision == WCSPH_PREC_MIXED;
+    ReorderArgs<D, Extra> a{};
+    a.perm = s->perm;
+    a.key = s->key;
+    a.start = s->start;
+    a.src = s->m[s->cur];
+    a.dst = s->m[1 - s->cur];
+    a.acc_old = s->acc;
+    a.s = s->sc;
+    a.step = kick;
+    a.keep_vel = s->cfg.precision == WCSPH_PREC_FP64 ? 1 : 0;
+    a.pk = s->pk;
+    a.g = s->grid;
+    a.pr64 = s->pr64;
+    a.c64 = s->c64;
+    a.e = EosConsts{s->p.rho0, B, s->p.gamma};
+    a.v_scale = mixed ? (float)(1.0 / s->p.c0) : 1.0f;
+    a.h_const = s->h_const;
+    a.extra = extra;
+    a.n = s->n;
+    return a;
+}
+
+template <int D, class Extra>
+static void launch_reorder(const wcsph_sim* s, const ReorderArgs<D, Extra>& a) {
+    hipLaunchKernelGGL((k_reorder<D, Extra>), dim3(s->nb(a.n)), dim3(kBlock), 0, s->st, a.perm, a.key, a.start,
+                       a.src, a.dst, a.acc_old, a.s, a.step, a.keep_vel, a.pk, a.g, a.pr64, a.c64, a.e, a.v_scale,
+                       a.h_const, a.extra, a.n);
+}
+
+template <int D>
+static void bin_and_reorder_d(wcsph_sim* s, int kick) {
+    const BinArgs<D> b = bin_args<D>(s, kick);
+    hipLaunchKernelGGL(k_bin<D>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, b.m, b.acc, b.divv, b.kick, b.n, b.g,
+                       b.key, b.slot, b.count, b.s);
+    const ScanArgs c = scan_args(s);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(s->ngroups), dim3(kBlock), 0, s->st, c.count, c.m, c.tile_total,
+                       c.group_total, c.gsize);
+    hipLaunchKernelGGL(k_scan_apply, dim3(s->npart), dim3(kBlock), 0, s->st, c.count, c.m, c.tile_total,
+                       c.group_total, c.gsize, c.start);
+    const ScatterArgs t = scatter_args(s);
+    hipLaunchKernelGGL(k_scatter, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, t.key, t.slot, t.start, t.perm, t.count,
+                       t.n);
+    if (s->cfg.precision == WCSPH_PREC_MIXED)
+        launch_reorder<D>(s, reorder_args<D>(s, kick, MixedExtra<D>{s->mx, s->mc}));
+    else
+        launch_reorder<D>(s, reorder_args<D>(s, kick, NoExtra{}));
+}
+
 // Cell list by counting sort and the gather into cell order. step: the
 // first half of a step (kick and drift) runs inside the binning pass, and
 // the reorder forms the predicted velocities.
 static void bin_and_reorder(wcsph_sim* s, bool step) {
-    const int64_t n = s->n;
-    Master& src = s->m[s->cur];
-    Master& dst = s->m[1 - s->cur];
     const int kick = step ? 1 : 0;
-    if (s->D == 3)
-        hipLaunchKernelGGL(k_bin<3>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src, s->acc, s->divv, kick, n, s->grid,
-                           s->key, s->slot, s->count, s->sc);
-    else
-        hipLaunchKernelGGL(k_bin<2>, dim3(s->nb(n)), dim3(kBlock), 0, s->st, src, s->acc, s->divv, kick, n, s->grid,
-                           s->key, s->slot, s->count, s->sc);
-    const int64_t gsize = scan_group_size(s->npart);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(s->ngroups), dim3(kBlock), 0, s->st, s->count, s->ncell + 1, s->tile_total,
-                       s->group_total, gsize);
-    hipLaunchKernelGGL(k_scan_apply, dim3(s->npart), dim3(kBlock), 0, s->st, s->count, s->ncell + 1, s->tile_total,
-                       s->group_total, gsize, s->start);
-    hipLaunchKernelGGL(k_scatter, dim3(s->nb(n)), dim3(kBlock), 0, s->st, s->key, s->slot, s->start, s->perm, s->count,
-                       n);
-    const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
-    EosConsts e{s->p.rho0, B, s->p.gamma};
-    const bool mixed = s->cfg.precision == WCSPH_PREC_MIXED;
-    const int keep_vel = s->cfg.precision == WCSPH_PREC_FP64 ? 1 : 0;
-    const float vscale = mixed ? (float)(1.0 / s->p.c0) : 1.0f;
-    const dim3 grid(s->nb(n)), block(kBlock);
-    if (s->D == 3) {
-        if (mixed)
-            hipLaunchKernelGGL((k_reorder<3, MixedExtra<3>>), grid, block, 0, s->st, s->perm, s->key, s->start, src,
-                               dst, s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale,
-                               s->h_const, MixedExtra<3>{s->mx, s->mc}, n);
-        else
-            hipLaunchKernelGGL((k_reorder<3, NoExtra>), grid, block, 0, s->st, s->perm, s->key, s->start, src, dst,
-                               s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale, s->h_const,
-                               NoExtra{}, n);
-    } else {
-        if (mixed)
-            hipLaunchKernelGGL((k_reorder<2, MixedExtra<2>>), grid, block, 0, s->st, s->perm, s->key, s->start, src,
-                               dst, s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale,
-                               s->h_const, MixedExtra<2>{s->mx, s->mc}, n);
-        else
-            hipLaunchKernelGGL((k_reorder<2, NoExtra>), grid, block, 0, s->st, s->perm, s->key, s->start, src, dst,
-                               s->acc, s->sc, kick, keep_vel, s->pk, s->grid, s->pr64, s->c64, e, vscale, s->h_const,
-                               NoExtra{}, n);
-    }
+    if (s->D == 3) bin_and_reorder_d<3>(s, kick);
+    else bin_and_reorder_d<2>(s, kick);
     s->cur = 1 - s->cur;
 }
 
@@ -409,8 +519,8 @@ static void nforce_dispatch_h(wcsph_sim* s) {
     else nforce_dispatch_hv<D, UH, 0>(s);
 }
 
-template <int D, int R, int VS, int BT>
-static void launch_nforce_mixed(wcsph_sim* s) {
+// Arguments of the mixed force pass over the current state.
+static NFMixedArgs nforce_mixed_args(const wcsph_sim* s) {
     NFMixedArgs a;
     a.mx = s->mx;
     a.pos = s->m[s->cur].pos;
@@ -437,7 +547,13 @@ static void launch_nforce_mixed(wcsph_sim* s) {
     a.s = s->sc;
     a.fused_final = s->fuse_final ? 1 : 0;
     a.fin = final_args(s, 1);
-    hipLaunchKernelGGL((k_nforce_mixed<D, R, VS, BT>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
+    return a;
+}
+
+template <int D, int R, int VS, int BT>
+static void launch_nforce_mixed(wcsph_sim* s) {
+    hipLaunchKernelGGL((k_nforce_mixed<D, R, VS, BT>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st,
+                       nforce_mixed_args(s));
 }
 
 template <int D, int BT>
@@ -562,7 +678,6 @@ static void one_step(wcsph_sim* s, bool filter) {
 }
 
 // FP64 diagnostics: per-block fixed-order tree sums, then one ordered pass.
-constexpr int kDiag = 10;
 
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_diag(Master m, int64_t n, double g, double rho0, double B,
@@ -624,16 +739,35 @@ extern "C" {
 
 int64_t wcsph_sim_bytes(int32_t dim, int64_t n, const wcsph_sim_config* cfg) {
     const int K = (cfg && cfg->max_neighbours > 0) ? cfg->max_neighbours : 128;
-    return sim_bytes(dim, n, K, n / 8);
+    const int prec = cfg ? cfg->precision : WCSPH_PREC_FP32;
+    return sim_footprint(dim, n, prec, K, n);  // a grid of about one cell per particle
+}
+
+int64_t wcsph_sim_bytes_state(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t n, const double* pos,
+                              const double* h) {
+    if (!p || !pos || !h || n <= 0 || (p->dim != 2 && p->dim != 3)) return -1;
+    const int K = (cfg && cfg->max_neighbours > 0) ? cfg->max_neighbours : 128;
+    const int prec = cfg ? cfg->precision : WCSPH_PREC_FP32;
+    return sim_footprint(p->dim, n, prec, K, sim_grid_cells(p, cfg, n, pos, h));
 }
 
 int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t n, wcsph_sim** out) {
     return guarded([&] {
-        check_params(p);
         WCSPH_REQUIRE(out != nullptr, WCSPH_EINVAL, "null output handle");
-        WCSPH_REQUIRE(n > 0 && n < ((int64_t)1 << 31), WCSPH_EINVAL, "n must be in [1, 2^31)");
         *out = nullptr;
+        *out = sim_create_in(p, cfg, n, nullptr, nullptr, nullptr);
+    });
+}
+
+}  // extern "C"
+
+wcsph_sim* wcsph::sim_create_in(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t n, wcsph_arena* arena,
+                                hipStream_t stream, volatile uint32_t* flags) {
+    {
+        check_params(p);
+        WCSPH_REQUIRE(n > 0 && n < ((int64_t)1 << 31), WCSPH_EINVAL, "n must be in [1, 2^31)");
         auto s = std::make_unique<wcsph_sim>();
+        s->arena = arena;
         s->p = *p;
         if (cfg) s->cfg = *cfg;
         WCSPH_REQUIRE(s->cfg.precision == WCSPH_PREC_FP32 || s->cfg.precision == WCSPH_PREC_FP64 ||
@@ -651,7 +785,12 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         WCSPH_REQUIRE(n < ((int64_t)1 << 27), WCSPH_EINVAL, "too many particles for one simulation (limit 2^27)");
         s->device = p->device;
         WCSPH_HIP(hipSetDevice(s->device));
-        WCSPH_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        if (stream) {
+            s->st = stream;
+            s->own_stream = false;
+        } else {
+            WCSPH_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        }
         for (auto& e : s->ev) WCSPH_HIP(hipEventCreate(&e));
         const int D = s->D;
         for (int b = 0; b < 2; ++b) {
@@ -702,12 +841,19 @@ int wcsph_sim_create(const wcsph_params* p, const wcsph_sim_config* cfg, int64_t
         WCSPH_HIP(hipMemsetAsync(s->acc, 0, sizeof(double) * D * n, s->st));
         WCSPH_HIP(hipMemsetAsync(s->divv, 0, sizeof(double) * n, s->st));
         WCSPH_HIP(hipMemsetAsync(s->sc, 0, sizeof(Scalars), s->st));
-        WCSPH_HIP(hipHostMalloc((void**)&s->hflags, 2 * sizeof(uint32_t), hipHostMallocDefault));
+        if (flags) {
+            s->hflags = flags;
+            s->own_flags = false;
+        } else {
+            WCSPH_HIP(hipHostMalloc((void**)&s->hflags, 2 * sizeof(uint32_t), hipHostMallocDefault));
+        }
         s->hflags[0] = s->hflags[1] = 0u;
         WCSPH_HIP(hipStreamSynchronize(s->st));
-        *out = s.release();
-    });
+        return s.release();
+    }
 }
+
+extern "C" {
 
 int wcsph_sim_destroy(wcsph_sim* s) {
     return guarded([&] { delete s; });
@@ -1317,3 +1463,44 @@ int wcsph_sim_restore(const char* path, int32_t device, wcsph_sim** out) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Support of the grouped steps (group.hip)
+
+template <int D>
+MixedStep<D> wcsph::mixed_step_args(wcsph_sim* s) {
+    MixedStep<D> a;
+    const int cur0 = s->cur;
+    a.sc = scalars_args(s, 1);
+    a.bin = bin_args<D>(s, 1);
+    a.scan = scan_args(s);
+    a.scat = scatter_args(s);
+    a.ro = reorder_args<D>(s, 1, MixedExtra<D>{s->mx, s->mc});
+    // the force pass reads the reordered state and ends with the final stage
+    s->cur = 1 - cur0;
+    s->fuse_final = true;
+    a.nf = nforce_mixed_args(s);
+    s->fuse_final = false;
+    s->cur = cur0;
+    a.nb = (int32_t)s->nb(s->n);
+    a.ngroups = (int32_t)s->ngroups;
+    a.npart = (int32_t)s->npart;
+    return a;
+}
+
+template MixedStep<2> wcsph::mixed_step_args<2>(wcsph_sim*);
+template MixedStep<3> wcsph::mixed_step_args<3>(wcsph_sim*);
+
+void wcsph::sim_one_step(wcsph_sim* s) {
+    WCSPH_HIP(hipSetDevice(s->device));
+    one_step(s, shepard_due(s, s->hstep));
+    ++s->hstep;
+    queue_flags(s);
+}
+
+void wcsph::sim_after_group_steps(wcsph_sim* s, int64_t steps) {
+    s->hstep += steps;
+    queue_flags(s);
+}
+
+void wcsph::sim_check_failed(const wcsph_sim* s) { raise_if_failed(s); }

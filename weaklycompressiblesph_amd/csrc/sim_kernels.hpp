@@ -98,8 +98,19 @@ struct Partials {
 #ifndef WCSPH_XCD_CHUNK
 #define WCSPH_XCD_CHUNK 64
 #endif
-__device__ __forceinline__ int xcd_block() {
-    const int b = blockIdx.x, nb = gridDim.x;
+// A block's index and the block count of the launch it belongs to: the
+// hardware's for an ordinary launch; for a grouped launch (one grid over
+// several simulations) the block's index among its simulation's blocks.
+struct VBlock {
+    int id, n;
+};
+
+__device__ __forceinline__ VBlock hw_block() { return VBlock{(int)blockIdx.x, (int)gridDim.x}; }
+
+// A grouped launch starts each simulation's blocks at a multiple of 8, so a
+// virtual block lands on the XCD its index names.
+__device__ __forceinline__ int xcd_block(VBlock vb) {
+    const int b = vb.id, nb = vb.n;
     constexpr int C = WCSPH_XCD_CHUNK;
     if (C > 0) {
         const int full = nb / (8 * C) * (8 * C);
@@ -110,6 +121,8 @@ __device__ __forceinline__ int xcd_block() {
     const int q = nb / 8, r = nb % 8, x = b % 8;
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + b / 8;
 }
+
+__device__ __forceinline__ int xcd_block() { return xcd_block(hw_block()); }
 
 template <int D>
 __device__ __forceinline__ void cell_of(const Grid& g, const double* x, int* c, bool& esc) {
@@ -132,8 +145,7 @@ __device__ __forceinline__ uint32_t cell_key(const Grid& g, const int* c) {
 // per-block bounds, time += dt, step += 1; 2: reset the per-pass counters
 // only. One block of kScalarsThreads threads, four loads in flight each.
 constexpr int kScalarsThreads = 1024;
-static __global__ __launch_bounds__(kScalarsThreads) void k_scalars(Scalars* s, const double* __restrict__ dtmin,
-                                                              int64_t nblk, int mode) {
+__device__ __forceinline__ void scalars_body(Scalars* s, const double* __restrict__ dtmin, int64_t nblk, int mode) {
     __shared__ double sh[kScalarsThreads / 64];
     if (mode == 1) {
         const int64_t T = kScalarsThreads;
@@ -180,6 +192,11 @@ static __global__ __launch_bounds__(kScalarsThreads) void k_scalars(Scalars* s, 
     s->maxcnt = 0;
 }
 
+static __global__ __launch_bounds__(kScalarsThreads) void k_scalars(Scalars* s, const double* __restrict__ dtmin,
+                                                              int64_t nblk, int mode) {
+    scalars_body(s, dtmin, nblk, mode);
+}
+
 // Counting sort, pass 1: cell key and slot within the cell, after the
 // first half of a step when kick != 0: initialKick and fullDrift
 // (src/class.h:146-155) in FP64 -- v_half = v + dt/2 a, rho += -rho dt div v,
@@ -190,10 +207,11 @@ static __global__ __launch_bounds__(kScalarsThreads) void k_scalars(Scalars* s, 
 // wave takes its slots with one atomic, made by the run's first lane and
 // broadcast to the rest.
 template <int D>
-__global__ void k_bin(Master m, const double* __restrict__ acc, const double* __restrict__ divv, int kick,
-                      int64_t n, Grid g, uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
-                      uint32_t* __restrict__ count, Scalars* s) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void bin_body(int64_t blk, Master m, const double* __restrict__ acc,
+                                         const double* __restrict__ divv, int kick, int64_t n, Grid g,
+                                         uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
+                                         uint32_t* __restrict__ count, Scalars* s) {
+    const int64_t i = blk * blockDim.x + threadIdx.x;
     const bool live = i < n;
     uint32_t k = 0xffffffffu;
     if (live) {
@@ -240,6 +258,13 @@ __global__ void k_bin(Master m, const double* __restrict__ acc, const double* __
     if (lead && live) base = atomicAdd(&count[k], (uint32_t)(next - lane));
     base = (uint32_t)__shfl((int)base, first, 64);
     if (live) slot[i] = base + (uint32_t)(lane - first);
+}
+
+template <int D>
+__global__ void k_bin(Master m, const double* __restrict__ acc, const double* __restrict__ divv, int kick,
+                      int64_t n, Grid g, uint32_t* __restrict__ key, uint32_t* __restrict__ slot,
+                      uint32_t* __restrict__ count, Scalars* s) {
+    bin_body<D>(blockIdx.x, m, acc, divv, kick, n, g, key, slot, count, s);
 }
 
 // Exclusive scan of count[0..m) into start[0..m], start[m] = total, over
@@ -296,13 +321,13 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* sh) {
     return total;
 }
 
-static __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t m,
-                                                                uint32_t* __restrict__ tile_total,
-                                                                uint32_t* __restrict__ group_total, int64_t gsize) {
+__device__ __forceinline__ void scan_reduce_body(int64_t blk, const uint32_t* __restrict__ in, int64_t m,
+                                                 uint32_t* __restrict__ tile_total,
+                                                 uint32_t* __restrict__ group_total, int64_t gsize) {
     __shared__ uint32_t sh[kBlock / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t ntiles = (m + kScanTile - 1) / kScanTile;
-    const int64_t t0 = (int64_t)blockIdx.x * gsize, t1 = t0 + gsize < ntiles ? t0 + gsize : ntiles;
+    const int64_t t0 = blk * gsize, t1 = t0 + gsize < ntiles ? t0 + gsize : ntiles;
     uint32_t mine = 0;  // this wave's share of the group total (lane 0)
     for (int64_t t = t0 + wv; t < t1; t += kBlock / 64) {
         // lane l sums counts [t * kScanTile + 16 l, + 16)
@@ -325,15 +350,21 @@ static __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* _
         }
     }
     const uint32_t g = block_sum(lane == 0 ? mine : 0u, sh);
-    if (threadIdx.x == 0) group_total[blockIdx.x] = g;
+    if (threadIdx.x == 0) group_total[blk] = g;
 }
 
-static __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ count, int64_t m,
-                                                               const uint32_t* __restrict__ tile_total,
-                                                               const uint32_t* __restrict__ group_total,
-                                                               int64_t gsize, uint32_t* __restrict__ start) {
+static __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t m,
+                                                                uint32_t* __restrict__ tile_total,
+                                                                uint32_t* __restrict__ group_total, int64_t gsize) {
+    scan_reduce_body(blockIdx.x, in, m, tile_total, group_total, gsize);
+}
+
+__device__ __forceinline__ void scan_apply_body(int64_t blk, const uint32_t* __restrict__ count, int64_t m,
+                                                const uint32_t* __restrict__ tile_total,
+                                                const uint32_t* __restrict__ group_total, int64_t gsize,
+                                                uint32_t* __restrict__ start) {
     __shared__ uint32_t sh[kBlock / 64];
-    const int64_t tile = blockIdx.x, g = tile / gsize;
+    const int64_t tile = blk, g = tile / gsize;
     uint32_t pre = 0;
     for (int64_t k = threadIdx.x; k < g; k += kBlock) pre += group_total[k];
     for (int64_t k = g * gsize + threadIdx.x; k < tile; k += kBlock) pre += tile_total[k];
@@ -355,17 +386,30 @@ static __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __
     }
 }
 
+static __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ count, int64_t m,
+                                                               const uint32_t* __restrict__ tile_total,
+                                                               const uint32_t* __restrict__ group_total,
+                                                               int64_t gsize, uint32_t* __restrict__ start) {
+    scan_apply_body(blockIdx.x, count, m, tile_total, group_total, gsize, start);
+}
+
 // Counting sort, pass 2: scatter old indices to their cell slots, and
 // clear the counts the scan has consumed for the next binning pass (only
 // occupied cells were counted, so only they need clearing).
-static __global__ void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
-                                 const uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
-                                 uint32_t* __restrict__ count, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void scatter_body(int64_t blk, const uint32_t* __restrict__ key,
+                                             const uint32_t* __restrict__ slot, const uint32_t* __restrict__ start,
+                                             uint32_t* __restrict__ perm, uint32_t* __restrict__ count, int64_t n) {
+    const int64_t i = blk * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = key[i];
     perm[start[k] + slot[i]] = (uint32_t)i;
     count[k] = 0u;
+}
+
+static __global__ void k_scatter(const uint32_t* __restrict__ key, const uint32_t* __restrict__ slot,
+                                 const uint32_t* __restrict__ start, uint32_t* __restrict__ perm,
+                                 uint32_t* __restrict__ count, int64_t n) {
+    scatter_body(blockIdx.x, key, slot, start, perm, count, n);
 }
 
 struct EosConsts {
@@ -454,14 +498,13 @@ struct Packed {  // FP32 force records in cell order
 // step == 0: velocities are copied as they are.
 // pr64/c64 (P/rho^2 and c in FP64) are written only for the FP64 path.
 template <int D, class Extra>
-__global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__ perm,
-                                                    const uint32_t* __restrict__ key,
-                                                    const uint32_t* __restrict__ start, Master src, Master dst,
-                                                    const double* __restrict__ acc_old, Scalars* s, int step,
-                                                    int keep_vel, Packed pk, Grid g, double* __restrict__ pr64,
-                                                    double* __restrict__ c64, EosConsts e, float v_scale,
-                                                    double h_const, Extra extra, int64_t n) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void reorder_body(int64_t blk, const uint32_t* __restrict__ perm,
+                                             const uint32_t* __restrict__ key, const uint32_t* __restrict__ start,
+                                             Master src, Master dst, const double* __restrict__ acc_old,
+                                             Scalars* s, int step, int keep_vel, Packed pk, Grid g,
+                                             double* __restrict__ pr64, double* __restrict__ c64, EosConsts e,
+                                             float v_scale, double h_const, Extra extra, int64_t n) {
+    const int64_t p = blk * blockDim.x + threadIdx.x;
     const bool live = p < n;
     float xl[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
     int c[3] = {0, 0, 0};
@@ -517,6 +560,18 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
         }
     }
     extra.store_pair(p, live, xl, c[0], vf, rho, pr, cs, m);  // every lane takes part
+}
+
+template <int D, class Extra>
+__global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__ perm,
+                                                    const uint32_t* __restrict__ key,
+                                                    const uint32_t* __restrict__ start, Master src, Master dst,
+                                                    const double* __restrict__ acc_old, Scalars* s, int step,
+                                                    int keep_vel, Packed pk, Grid g, double* __restrict__ pr64,
+                                                    double* __restrict__ c64, EosConsts e, float v_scale,
+                                                    double h_const, Extra extra, int64_t n) {
+    reorder_body<D, Extra>(blockIdx.x, perm, key, start, src, dst, acc_old, s, step, keep_vel, pk, g, pr64, c64, e,
+                           v_scale, h_const, extra, n);
 }
 
 struct NoExtra {  // FP32 and FP64 paths: the FP32 records are all there is
