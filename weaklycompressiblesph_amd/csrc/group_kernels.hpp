@@ -101,6 +101,7 @@ struct ReorderArgs {
     EosConsts e;
     float v_scale;
     double h_const;
+    double m_const;
     Extra extra;
     int64_t n;
 };
@@ -150,7 +151,8 @@ __global__ __launch_bounds__(kBlock) void kg_reorder(ctab<ReorderArgs<D, Extra>>
     if (b >= gm.nblk[k]) return;
     const ReorderArgs<D, Extra>& a = args_at(t + k);
     reorder_body<D, Extra>(b, a.perm, a.key, a.start, a.src, a.dst, a.acc_old, a.s, a.step, a.keep_vel, a.pk, a.g,
-                           a.pr64, a.c64, a.e, a.v_scale, a.h_const, a.extra, a.n);
+                           a.pr64, a.c64, a.e, a.v_scale, a.h_const, a.m_const, a.extra,
+                           a.n);
 }
 
 template <int D, int R, int VS, int BT>

@@ -237,6 +237,13 @@ static void raise_if_failed(const wcsph_sim* s) {
     throw Error{WCSPH_EDIVERGED, msg};
 }
 
+// v[0] when all n values are equal and positive, else 0.
+static double uniform_value(int64_t n, const double* v) {
+    for (int64_t i = 1; i < n; ++i)
+        if (v[i] != v[0]) return 0.0;
+    return v[0] > 0.0 ? v[0] : 0.0;
+}
+
 // Largest smoothing length (as FP32, the precision of the predicate) and
 // whether all are equal.
 static void scan_h(wcsph_sim* s, const double* h) {
@@ -340,6 +347,7 @@ static ReorderArgs<D, Extra> reorder_args(const wcsph_sim* s, int kick, Extra ex
     a.e = EosConsts{s->p.rho0, B, s->p.gamma};
     a.v_scale = mixed ? (float)(1.0 / s->p.c0) : 1.0f;
     a.h_const = s->h_const;
+    a.m_const = s->m_const;
     a.extra = extra;
     a.n = s->n;
     return a;
@@ -349,7 +357,7 @@ template <int D, class Extra>
 static void launch_reorder(const wcsph_sim* s, const ReorderArgs<D, Extra>& a) {
     hipLaunchKernelGGL((k_reorder<D, Extra>), dim3(s->nb(a.n)), dim3(kBlock), 0, s->st, a.perm, a.key, a.start,
                        a.src, a.dst, a.acc_old, a.s, a.step, a.keep_vel, a.pk, a.g, a.pr64, a.c64, a.e, a.v_scale,
-                       a.h_const, a.extra, a.n);
+                       a.h_const, a.m_const, a.extra, a.n);
 }
 
 template <int D>
@@ -894,6 +902,9 @@ int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const d
         WCSPH_HIP(hipMemcpy(m.h, h, sizeof(double) * n, hipMemcpyHostToDevice));
         if (s->h_const > 0.0)  // the reorder does not permute a constant h: both buffers hold it
             WCSPH_HIP(hipMemcpy(s->m[1 - s->cur].h, h, sizeof(double) * n, hipMemcpyHostToDevice));
+        s->m_const = uniform_value(n, mass);  // nor a constant mass
+        if (s->m_const > 0.0)
+            WCSPH_HIP(hipMemcpy(s->m[1 - s->cur].mass, mass, sizeof(double) * n, hipMemcpyHostToDevice));
         WCSPH_HIP(hipMemcpy(m.type, t8.data(), n, hipMemcpyHostToDevice));
         WCSPH_HIP(hipMemcpy(m.id, id.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
         s->uploaded = true;
@@ -1435,6 +1446,8 @@ int wcsph_sim_restore(const char* path, int32_t device, wcsph_sim** out) {
         put(m.mass, hs.mass.data(), sizeof(double) * n);
         put(m.h, hs.h.data(), sizeof(double) * n);
         if (s->h_const > 0.0) put(s->m[1 - s->cur].h, hs.h.data(), sizeof(double) * n);
+        s->m_const = uniform_value(n, hs.mass.data());
+        if (s->m_const > 0.0) put(s->m[1 - s->cur].mass, hs.mass.data(), sizeof(double) * n);
         put(s->acc, hs.acc.data(), sizeof(double) * D * n);
         put(s->divv, hs.divv.data(), sizeof(double) * n);
         put(s->dti, hs.dti.data(), sizeof(double) * n);

@@ -41,6 +41,7 @@ struct wcsph_sim {
     float hmax = 0.f;
     bool uniform_h = false;  // all smoothing lengths equal in FP32 (fixed in time)
     double h_const = 0.0;    // > 0: all smoothing lengths are exactly this (kept in both buffers, not permuted)
+    double m_const = 0.0;    // > 0: all masses are exactly this (kept in both buffers, not permuted)
     double *pr64 = nullptr, *c64 = nullptr;  // FP64 path only
     Mixed mx{};                               // mixed path only
     MixedConsts mc{};

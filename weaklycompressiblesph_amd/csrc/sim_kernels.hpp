@@ -503,7 +503,8 @@ __device__ __forceinline__ void reorder_body(int64_t blk, const uint32_t* __rest
                                              Master src, Master dst, const double* __restrict__ acc_old,
                                              Scalars* s, int step, int keep_vel, Packed pk, Grid g,
                                              double* __restrict__ pr64, double* __restrict__ c64, EosConsts e,
-                                             float v_scale, double h_const, Extra extra, int64_t n) {
+                                             float v_scale, double h_const, double m_const, Extra extra,
+                                             int64_t n) {
     const int64_t p = blk * blockDim.x + threadIdx.x;
     const bool live = p < n;
     float xl[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
@@ -538,12 +539,12 @@ __device__ __forceinline__ void reorder_body(int64_t blk, const uint32_t* __rest
             vf[d] = (float)(v * v_scale);
         }
         rho = src.dens[o];
-        m = src.mass[o];
-        // h_const > 0: every smoothing length is h_const, in both buffers,
-        // so the permutation leaves h as it is
+        // h_const > 0 (m_const > 0): every smoothing length (mass) is that
+        // constant, in both buffers, so the permutation leaves it as it is
+        m = m_const > 0.0 ? m_const : src.mass[o];
         const double h = h_const > 0.0 ? h_const : src.h[o];
         dst.dens[p] = rho;
-        dst.mass[p] = m;
+        if (!(m_const > 0.0)) dst.mass[p] = m;
         if (!(h_const > 0.0)) dst.h[p] = h;
         dst.type[p] = src.type[o];
         dst.id[p] = src.id[o];
@@ -569,9 +570,9 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
                                                     const double* __restrict__ acc_old, Scalars* s, int step,
                                                     int keep_vel, Packed pk, Grid g, double* __restrict__ pr64,
                                                     double* __restrict__ c64, EosConsts e, float v_scale,
-                                                    double h_const, Extra extra, int64_t n) {
+                                                    double h_const, double m_const, Extra extra, int64_t n) {
     reorder_body<D, Extra>(blockIdx.x, perm, key, start, src, dst, acc_old, s, step, keep_vel, pk, g, pr64, c64, e,
-                           v_scale, h_const, extra, n);
+                           v_scale, h_const, m_const, extra, n);
 }
 
 struct NoExtra {  // FP32 and FP64 paths: the FP32 records are all there is
