@@ -450,6 +450,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 skips it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cells-per-support", type=int, default=0, help="1 or 2 (0: library default)")
+    ap.add_argument("--shepard", type=int, default=0,
+                    help="Shepard density filter every N steps (the reference uses 30; 0: off, the default)")
     ap.add_argument("--queue-depth", type=int, default=0, help="24 or 32 (0: library default)")
     ap.add_argument("--accuracy", action="store_true",
                     help="run the FP32 and mixed paths against the FP64 path to --t-end and print their errors")
@@ -498,7 +500,7 @@ def main():
         marks = [m for m in (0.25, 0.5, 1.0) if m <= args.t_end] or [args.t_end]
         print(json.dumps(accuracy_study(W, dim, dx, marks, 50)), flush=True)
         return
-    params = W.dam_break_params(dim)
+    params = W.dam_break_params(dim, shepard_interval=args.shepard)
     desc = W.dam_break_desc(dim, dx)
     st = W.dam_break_state(desc, params)
     n = len(st["dens"])
@@ -536,7 +538,7 @@ def main():
         "config": {"workload": f"{dim}D dam break, dx={dx} m, {n} particles ({st['n_fluid']} fluid), "
                                f"{args.precision} force path, full device-resident steps",
                    "particles": n, "dx": dx, "precision": args.precision,
-                   "cells_per_support": args.cells_per_support or 2,
+                   "cells_per_support": args.cells_per_support or 2, "shepard_interval": args.shepard,
                    "develop_steps": args.develop,
                    "step_window": f"steps {args.develop + args.warmup + 1}-{args.develop + args.warmup + K}"},
         "paths": runs,

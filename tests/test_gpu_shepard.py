@@ -1,8 +1,9 @@
 """GPU checks of the Shepard density filter (src/force.h:10-30) in the
 device-resident step: applied after the first force pass and on steps with
 step % shepard_interval == 0 (src/main.cpp:75,105), in FP64 from the master
-state. Compared with the oracle's filtered run (tests/test_oracle.py pins the
-oracle's filter)."""
+state on the FP64 path and in FP32 from the FP32 force records on the FP32
+and mixed paths. Compared with the oracle's filtered run (tests/test_oracle.py
+pins the oracle's filter)."""
 import numpy as np
 import pytest
 
@@ -25,12 +26,17 @@ def test_filter_after_init_matches_oracle(O, W, dim, dx, geom):
     op = O.sim_params(dim, alpha=p.alpha, c_cfl=p.c_cfl, c0=p.c0, g=p.g, nu=p.nu, shepard_interval=30)
     st0 = state_dict(s)
     s.init(op, 8)
+    # FP64: the oracle's arithmetic to rounding; FP32 (FP32 and mixed paths):
+    # distance, W and the sums in FP32 on cell-local positions, the density
+    # taken as rho0 / (1 + T / S) (rho - rho0 carries the information)
+    tol = {"fp64": 1e-12, "fp32": 1e-8, "mixed": 1e-8}
     for prec in ("fp64", "fp32", "mixed"):
         with W.Simulation.from_state(p, st0, precision=prec) as sim:
             sim.init()
             out = sim.download()
-        # the filter reads the FP64 master state on every path
-        assert np.allclose(out["dens"], s.dens, rtol=1e-12), prec
+        err = np.max(np.abs(out["dens"] - s.dens) / s.dens)
+        print(f"{dim}D {prec}: max relative density error after the filter {err:.2e}")
+        assert err <= tol[prec], prec
         assert not np.allclose(out["dens"], st0["dens"], rtol=1e-9)
 
 

@@ -625,40 +625,44 @@ static hipEvent_t* step_events(wcsph_sim* s) {
 }
 
 // Shepard filter over the current cell order (records must be up to date).
-static void shepard_pass(wcsph_sim* s) {
+// The Shepard filter: FP64 from the master state on the FP64 path, FP32
+// from the FP32 force records on the FP32 and mixed paths (the mixed path
+// writes them for the pass), then the commit of the filtered densities.
+template <int D>
+static void shepard_pass_d(wcsph_sim* s) {
     const Master& m = s->m[s->cur];
     const int64_t n = s->n;
-    const double cnorm = s->D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI;  // src/kernel.h:35-41
+    const dim3 grid(s->nb(n)), block(kBlock);
+    if (s->cfg.precision == WCSPH_PREC_FP64) {
+        const double cnorm = D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI;  // src/kernel.h:35-41
+        if (s->R == 1)
+            hipLaunchKernelGGL((k_shepard<D, 1>), grid, block, 0, s->st, m, s->pk, s->start, s->grid, n, cnorm,
+                               s->rho_tmp, s->sc);
+        else
+            hipLaunchKernelGGL((k_shepard<D, 2>), grid, block, 0, s->st, m, s->pk, s->start, s->grid, n, cnorm,
+                               s->rho_tmp, s->sc);
+    } else {
+        if (s->cfg.precision == WCSPH_PREC_MIXED) fp32_records(s);
+        if (s->R == 1)
+            hipLaunchKernelGGL((k_shepard32<D, 1>), grid, block, 0, s->st, s->pk, s->start, s->grid, n, s->p.rho0,
+                               m.dens, s->rho_tmp, s->sc);
+        else
+            hipLaunchKernelGGL((k_shepard32<D, 2>), grid, block, 0, s->st, s->pk, s->start, s->grid, n, s->p.rho0,
+                               m.dens, s->rho_tmp, s->sc);
+    }
     const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
     EosConsts e{s->p.rho0, B, s->p.gamma};
-    const bool mixed = s->cfg.precision == WCSPH_PREC_MIXED;
-    if (s->D == 3) {
-        if (s->R == 1)
-            hipLaunchKernelGGL((k_shepard<3, 1>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
-                               n, cnorm, s->rho_tmp, s->sc);
-        else
-            hipLaunchKernelGGL((k_shepard<3, 2>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
-                               n, cnorm, s->rho_tmp, s->sc);
-        if (mixed)
-            hipLaunchKernelGGL((k_apply_density<3, MixedExtra<3>>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk,
-                               s->rho_tmp, s->pr64, s->c64, e, MixedExtra<3>{s->mx, s->mc}, n);
-        else
-            hipLaunchKernelGGL((k_apply_density<3, NoExtra>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk,
-                               s->rho_tmp, s->pr64, s->c64, e, NoExtra{}, n);
-    } else {
-        if (s->R == 1)
-            hipLaunchKernelGGL((k_shepard<2, 1>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
-                               n, cnorm, s->rho_tmp, s->sc);
-        else
-            hipLaunchKernelGGL((k_shepard<2, 2>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk, s->start, s->grid,
-                               n, cnorm, s->rho_tmp, s->sc);
-        if (mixed)
-            hipLaunchKernelGGL((k_apply_density<2, MixedExtra<2>>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk,
-                               s->rho_tmp, s->pr64, s->c64, e, MixedExtra<2>{s->mx, s->mc}, n);
-        else
-            hipLaunchKernelGGL((k_apply_density<2, NoExtra>), dim3(s->nb(n)), dim3(kBlock), 0, s->st, m, s->pk,
-                               s->rho_tmp, s->pr64, s->c64, e, NoExtra{}, n);
-    }
+    if (s->cfg.precision == WCSPH_PREC_MIXED)
+        hipLaunchKernelGGL((k_apply_density<D, MixedExtra<D>>), grid, block, 0, s->st, m, s->pk, s->rho_tmp,
+                           s->pr64, s->c64, e, MixedExtra<D>{s->mx, s->mc}, n);
+    else
+        hipLaunchKernelGGL((k_apply_density<D, NoExtra>), grid, block, 0, s->st, m, s->pk, s->rho_tmp, s->pr64,
+                           s->c64, e, NoExtra{}, n);
+}
+
+static void shepard_pass(wcsph_sim* s) {
+    if (s->D == 3) shepard_pass_d<3>(s);
+    else shepard_pass_d<2>(s);
 }
 
 static bool shepard_due(const wcsph_sim* s, int64_t step) {

@@ -1152,6 +1152,66 @@ __global__ __launch_bounds__(kBlock) void k_shepard(Master m, Packed pk, const u
     out[i] = sum / one;
 }
 
+// The same filter in FP32 arithmetic for the FP32 and mixed paths, from the
+// FP32 force records of the current cell order (cell-local positions, so
+// differences lose nothing to the magnitude of the coordinates). The
+// density carries its information in rho - rho0 (a few parts in 1e3), so
+// the filter is taken in that form: with q_j = rho0 / rho_j - 1 (formed as
+// (rho0 - rho_j) / rho_j, the difference exact in FP64),
+//   sum_j (m_j / rho_j) W_ij = (S + T) / rho0,  S = sum m_j W_ij,
+//   T = sum m_j W_ij q_j,  so  rho_i' = rho0 / (1 + T / S),
+// and FP32 rounding of S and T moves T / S, and rho' - rho0, only by parts
+// in 1e7. W's normalisation is common to both sums and left out.
+template <int D, int R>
+__global__ __launch_bounds__(kBlock) void k_shepard32(Packed pk, const uint32_t* __restrict__ start, Grid g,
+                                                      int64_t n, double rho0, const double* __restrict__ dens,
+                                                      double* __restrict__ out, const Scalars* sc) {
+    const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (sim_failed(sc)) {  // a diverged state keeps its densities
+        out[i] = dens[i];
+        return;
+    }
+    const float4 xi4 = pk.px[i];
+    const float xi[3] = {xi4.x, xi4.y, xi4.z};
+    const float H = 2.f * pk.pq[i].w, H2 = H * H, invH = 1.f / H;
+    int c[3];
+    key_cell<D>(g, pk.pcell[i], c);
+    const float hc = H * 1.0001f;
+    Stencil<D, R> st;
+    st.k = -1;
+    st.advance(g, start, c, xi, hc * hc);
+    float S = 0.f, T = 0.f;
+    while (st.k < Stencil<D, R>::NS) {
+        float dc[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) dc[d] = Stencil<D, R>::dcell(st.k, d);
+        for (uint32_t j = st.p; j < st.pend; ++j) {
+            const float4 pj = pk.px[j];
+            const float pjv[3] = {pj.x, pj.y, pj.z};
+            float r2 = 0.f;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const float dd = fmaf(dc[d], g.Lf, xi[d]) - pjv[d];
+                r2 += dd * dd;
+            }
+            if (r2 < H2) {
+                const float s = sqrtf(r2) * invH;
+                const float a = 1.f - s, b = 0.5f - s;
+                float w = a * a * a;
+                if (b > 0.f) w -= 4.f * b * b * b;
+                const double rj = dens[j];
+                const float q = (float)(rho0 - rj) / (float)rj;
+                const float mw = pk.pv[j].w * w;
+                S += mw;
+                T += mw * q;
+            }
+        }
+        st.advance(g, start, c, xi, hc * hc);
+    }
+    out[i] = rho0 / (1.0 + (double)T / (double)S);
+}
+
 // Commit the filtered density (copyFromForce, src/class.h:116-118) and redo
 // what k_reorder derived from it: P/rho^2 and c in the force records (and
 // the FP64 copies and the mixed state records when present).
