@@ -69,6 +69,13 @@ typedef struct {
     int32_t shepard_interval; /* device-resident step: Shepard density filter (src/force.h:10-30)
                                  at init and on steps with step % interval == 0
                                  (src/main.cpp:75,105; src/param.h:12 uses 30); 0 = off */
+    int32_t grad_mode;  /* 0: kernel gradient at the mean smoothing length (src/force.h:49);
+                           1: the mean of the gradients at h_i and h_j, as the reference GPU
+                           kernel takes it (src/kernel.cu:265), a pair then interacting within
+                           2 max(h_i, h_j). The two agree when h is uniform. The FDPS drop-in
+                           takes either; the device-resident paths and wcsph_force_pass take
+                           1 only with one smoothing length for all particles. */
+    int32_t pad_;
 } wcsph_params;
 
 /* Fill p with the reference CPU functor's constants (src/force.h:32,45-47,

@@ -29,7 +29,7 @@ i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
 
 class ForceParams(C.Structure):
     _fields_ = [("dim", C.c_int), ("vsig_mode", C.c_int), ("alpha", C.c_double),
-                ("beta", C.c_double), ("eps", C.c_double), ("c_cfl", C.c_double)]
+                ("beta", C.c_double), ("eps", C.c_double), ("c_cfl", C.c_double), ("grad_mode", C.c_int)]
 
 
 class SimParams(C.Structure):
@@ -144,9 +144,10 @@ def res_dtype(dim):
     return np.dtype([("acc", "<f8", (dim,)), ("dt", "<f8"), ("div_v", "<f8")])
 
 
-def force_params(dim, alpha=0.01, beta=0.0, eps=0.01, c_cfl=0.3, vsig_mode=0):
-    """Defaults are the reference CPU functor's constants (src/force.h:32,45-47, src/param.h:5)."""
-    return ForceParams(dim, vsig_mode, alpha, beta, eps, c_cfl)
+def force_params(dim, alpha=0.01, beta=0.0, eps=0.01, c_cfl=0.3, vsig_mode=0, grad_mode=0):
+    """Defaults are the reference CPU functor's constants (src/force.h:32,45-47, src/param.h:5);
+    grad_mode 1 takes the gradient as the reference GPU kernel does (src/kernel.cu:265)."""
+    return ForceParams(dim, vsig_mode, alpha, beta, eps, c_cfl, grad_mode)
 
 
 def kernel_W(dim, r, h):

@@ -81,8 +81,12 @@ static int pair_term(const orc_force_params* p, int D, const orc_pv* I, const or
     const double AV = (-p->alpha * c_ij * mu + p->beta * mu * mu) / dens_ij;      /* :46 */
     const double vs = p->vsig_mode ? -mu : mu;                                      /* :47 */
     if (vs > *vsig) *vsig = vs;
-    /* :49 gradW at the mean smoothing length */
-    const double F = orc_kernel_gradW_factor(D, sqrt(r2), 0.5 * (I->smth + J->smth));
+    /* :49 gradW at the mean smoothing length; grad_mode 1: the mean of the
+     * gradients at h_i and h_j, as the reference GPU kernel takes it
+     * (src/kernel.cu:265; src/force.h:48 has it commented out) */
+    const double r = sqrt(r2);
+    const double F = p->grad_mode ? 0.5 * (orc_kernel_gradW_factor(D, r, I->smth) + orc_kernel_gradW_factor(D, r, J->smth))
+                                  : orc_kernel_gradW_factor(D, r, 0.5 * (I->smth + J->smth));
     const double pt = I->pres / (I->dens * I->dens) + J->pres / (J->dens * J->dens) + AV;
     double drgw = 0, dvgw = 0;
     for (int k = 0; k < D; ++k) {
@@ -526,7 +530,7 @@ static void state_pv(const orc_sim_params* p, const orc_state* s, int64_t i,
 int64_t orc_sim_force(const orc_sim_params* p, orc_state* s, int nthreads) {
     const int D = p->dim;
     const int64_t n = s->n;
-    orc_force_params fp = {D, p->vsig_mode, p->alpha, p->beta, p->eps, p->c_cfl};
+    orc_force_params fp = {D, p->vsig_mode, p->alpha, p->beta, p->eps, p->c_cfl, 0};
     double hmax = 0, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     for (int64_t i = 0; i < n; ++i) if (s->h[i] > hmax) hmax = s->h[i];
     for (int k = 0; k < D; ++k) {

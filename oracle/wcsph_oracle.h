@@ -35,6 +35,8 @@ typedef struct {
     double beta;      /* quadratic AV coefficient (0 in src/force.h:46) */
     double eps;       /* regulariser 0.01 in src/force.h:45,53 */
     double c_cfl;     /* src/param.h:5 */
+    int grad_mode;    /* 0: gradW at the mean smoothing length (src/force.h:49);
+                         1: the mean of the gradients at h_i and h_j (src/kernel.cu:265) */
 } orc_force_params;
 
 /* src/kernel.h:27-55 (CubicSpline<D>) */

@@ -89,6 +89,10 @@ CASES = [
     (2, 0.02, 40, "config"),
     (3, 0.04, 30, "reference"),
     (3, 0.04, 30, "config-beta-vsig"),
+    # grad_mode 1: the reference GPU kernel's mean of the gradients at h_i
+    # and h_j (src/kernel.cu:265), with per-particle h varying by 10%
+    (2, 0.02, 40, "config-grad"),
+    (3, 0.04, 30, "config-grad-vsig"),
 ]
 
 
@@ -100,8 +104,9 @@ def test_dispatch_retrieve_matches_oracle(O, W, dim, dx, steps, pset):
         q = W.reference_params(dim)
     else:
         q = W.dam_break_params(dim, alpha=0.1, beta=0.2 if "beta" in pset else 0.0,
-                               vsig_mode=1 if "vsig" in pset else 0)
-    fp = O.force_params(dim, alpha=q.alpha, beta=q.beta, eps=q.eps, c_cfl=q.c_cfl, vsig_mode=q.vsig_mode)
+                               vsig_mode=1 if "vsig" in pset else 0, grad_mode=1 if "grad" in pset else 0)
+    fp = O.force_params(dim, alpha=q.alpha, beta=q.beta, eps=q.eps, c_cfl=q.c_cfl, vsig_mode=q.vsig_mode,
+                        grad_mode=q.grad_mode)
     epi_all, epj_all = records(O, s, p.nu, rng if pset != "reference" else None)
     epi, epj = walks(O, epi_all, epj_all, 128, rng)
     W.init(q)

@@ -506,3 +506,24 @@ def test_upload_rejects_nonfinite(W):
         with pytest.raises(W.WcsphError) as e:
             W.Simulation.from_state(p, bad)
         assert e.value.code == W._abi.WCSPH_EINVAL
+
+
+def test_grad_mode_device_paths(W):
+    # the device-resident paths take grad_mode 1 only with one smoothing
+    # length, where it is the same pair body as grad_mode 0 (same bits)
+    p0 = W.dam_break_params(2)
+    p1 = W.dam_break_params(2, grad_mode=1)
+    st = W.dam_break_state(W.dam_break_desc(2, 0.04), p0)
+    outs = []
+    for p in (p0, p1):
+        with W.Simulation.from_state(p, st, precision="fp32") as sim:
+            sim.init()
+            sim.step(10)
+            outs.append(sim.download())
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    st2 = dict(st)
+    st2["h"] = st["h"] * (1 + 0.05 * (np.arange(len(st["h"])) % 3))
+    for prec in ("fp32", "fp64"):
+        with pytest.raises(W.WcsphError, match="grad_mode 1"):
+            W.Simulation.from_state(p1, st2, precision=prec)

@@ -163,6 +163,53 @@ def test_momentum_antisymmetry(O, dim):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
+def test_grad_mode_pair_is_mean_of_single_h_pairs(O, dim):
+    # grad_mode 1 (src/kernel.cu:265): the pressure force of a pair at rest
+    # with h_i != h_j is the mean of the forces of the same pair with both
+    # smoothing lengths h_i and with both h_j (mode 0, pinned above by finite
+    # differences); the second pair sits beyond h_i + h_j but within
+    # 2 max(h_i, h_j), where mode 0 gives nothing and mode 1 half of one term
+    hs = (0.5, 0.6)
+    for r, beyond_mean in ((0.7, False), (1.15, True)):
+        epi, epj = make_cloud(O, dim, 2, seed=1, P=True, nu=0.0)
+        for a in (epi, epj):
+            a["vel"] = 0
+            a["pos"] = 0
+            a["pos"][1][0] = r
+            a["dens"] = [1010.0, 1004.0]
+            a["pres"] = [3.0e4, 1.5e4]
+
+        def acc(hpair, mode):
+            for a in (epi, epj):
+                a["smth"] = hpair
+            return O.hydro_force(O.force_params(dim, grad_mode=mode), epi, epj)["acc"][0][0]
+
+        a1 = acc(hs, 1)
+        expect = 0.5 * (acc((hs[0], hs[0]), 0) + acc((hs[1], hs[1]), 0))
+        assert a1 == pytest.approx(expect, rel=1e-13)
+        assert (acc(hs, 0) == 0.0) == beyond_mean
+        assert a1 < 0  # repulsion either way
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_grad_mode_uniform_h_and_antisymmetry(O, dim):
+    # with one smoothing length the two gradient forms are the same numbers;
+    # with varying h the mean of the gradients is symmetric in (i, j), so
+    # sum_i m_i a_i = 0 (taking only the gradient at h_i breaks it)
+    epi, epj = make_cloud(O, dim, 200, hvar=0.0, mvar=0.3, seed=5)
+    r0 = O.hydro_force(O.force_params(dim, alpha=0.3), epi, epj)
+    r1 = O.hydro_force(O.force_params(dim, alpha=0.3, grad_mode=1), epi, epj)
+    assert np.array_equal(r0, r1)
+    epi, epj = make_cloud(O, dim, 300, hvar=0.3, mvar=0.3, seed=6)
+    res = O.hydro_force(O.force_params(dim, alpha=0.3, beta=0.5, grad_mode=1), epi, epj)
+    ma = (epj["mass"][:, None] * res["acc"]).sum(0)
+    scale = (epj["mass"][:, None] * np.abs(res["acc"])).sum(0)
+    assert np.all(np.abs(ma) < 1e-12 * scale.max())
+    res0 = O.hydro_force(O.force_params(dim, alpha=0.3, beta=0.5), epi, epj)
+    assert not np.allclose(res["acc"], res0["acc"], rtol=1e-6)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
 def test_dissipation_sign(O, dim):
     # with P = 0 only artificial and laminar viscosity act: they must remove
     # kinetic energy, sum_i m_i v_i.a_i <= 0

@@ -175,9 +175,8 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk_force(WalkArgs<D> a) {
                     float mu;
                     (void)pair_mu(r2, drdv, I, pj.w, a.k, mu, vsig);
                 }
-                const float H = I.h + pj.w;
                 lst[cnt][t] = (uint8_t)v;  // kept only if counted
-                cnt += r2 < H * H ? 1 : 0;
+                cnt += r2 < pair_support2(I.h, pj.w, a.k.grad_mode) ? 1 : 0;
             };
             while (u < nt) {
                 // scan in groups of kWalkScan candidates (their LDS reads in
@@ -608,6 +607,7 @@ void dispatch(int32_t tag, int32_t n_walk, const EPI* const* epi, const int32_t*
     a.k.eps = (float)B.p.eps;
     a.k.cnorm = (float)(D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI);
     a.k.vsig_mode = B.p.vsig_mode;
+    a.k.grad_mode = B.p.grad_mode;
     a.c_cfl = (float)B.p.c_cfl;
     WCSPH_HIP(hipEventRecord(T.ev_begin, T.copy_stream));
     double pack_ms = 0.0;
@@ -745,6 +745,7 @@ int backend_init(const wcsph_params* p) {
     return guarded([&] {
         WCSPH_REQUIRE(p != nullptr, WCSPH_EINVAL, "null params");
         WCSPH_REQUIRE(p->dim == 2 || p->dim == 3, WCSPH_EINVAL, "dim must be 2 or 3");
+        WCSPH_REQUIRE(p->grad_mode == 0 || p->grad_mode == 1, WCSPH_EINVAL, "grad_mode must be 0 or 1");
         Backend& B = backend();
         std::lock_guard<std::mutex> lk(B.mu);
         int ndev = 0;

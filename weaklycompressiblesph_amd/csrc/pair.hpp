@@ -9,6 +9,9 @@
 //   gradW = F dr,  F = C_D g(s) / (H^(D+1) r),  H = h_i + h_j, s = r / H
 //           g(s) = -3 (1-s)+^2 + 12 (1/2-s)+^2             (src/kernel.h:49-55
 //           evaluated at the mean smoothing length, src/force.h:49)
+//           grad_mode 1: F = (F(2 h_i) + F(2 h_j)) / 2, the reference GPU
+//           kernel's mean of the gradients (src/kernel.cu:265), nonzero
+//           within 2 max(h_i, h_j)
 //   acc  -= m_j (P_i/rho_i^2 + P_j/rho_j^2 + Pi) gradW
 //   acc  += m_j 2 (nu_i + nu_j) (dr.gradW) / (rho_i + rho_j) / den dv
 //   divv -= m_j dv.gradW
@@ -24,7 +27,27 @@ template <typename T>
 struct PairConsts {
     T alpha, beta, eps, cnorm;  // cnorm = C_D: 80/(7 pi) in 2D, 16/pi in 3D (src/kernel.h:35-41)
     int vsig_mode;
+    int grad_mode;  // 0: gradient at the mean h; 1: mean of the gradients at h_i and h_j
 };
+
+// Squared support radius of a pair: (h_i + h_j)^2, or (2 max(h_i, h_j))^2
+// when the gradient is the mean of the two particles' gradients.
+template <typename T>
+__device__ __forceinline__ T pair_support2(T hi, T hj, int grad_mode) {
+    const T H = grad_mode ? T(2) * (hi > hj ? hi : hj) : hi + hj;
+    return H * H;
+}
+
+// g(r / H) / H^(D+1) of the cubic spline (src/kernel.h:49-55) for 1/H = invH.
+template <int D, typename T>
+__device__ __forceinline__ T spline_gH(T r, T invH) {
+    const T s = r * invH;
+    const T a = s < T(1) ? T(1) - s : T(0);
+    const T b = s < T(0.5) ? T(0.5) - s : T(0);
+    const T g = T(-3) * a * a + T(12) * b * b;
+    const T invH2 = invH * invH;
+    return g * (D == 3 ? invH2 * invH2 : invH2 * invH);
+}
 
 __device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double fast_rcp(double x) { return 1.0 / x; }
@@ -62,16 +85,11 @@ __device__ __forceinline__ void pair_force(const T* dr, const T* dv, T r2, T drd
     const T inv_rs = fast_rcp(rho_s);
     const T cbar = T(0.5) * (I.c + J.c);
     const T Pi = (-k.alpha * cbar * mu + k.beta * mu * mu) * (T(2) * inv_rs);  // 1/rhobar = 2/(rho_i+rho_j)
-    const T H = I.h + J.h;
-    const T invH = fast_rcp(H);
     const T invr = fast_rsqrt(r2);
-    const T s = r2 * invr * invH;
-    const T a = s < T(1) ? T(1) - s : T(0);
-    const T b = s < T(0.5) ? T(0.5) - s : T(0);
-    const T g = T(-3) * a * a + T(12) * b * b;
-    const T invH2 = invH * invH;
-    const T invHD1 = D == 3 ? invH2 * invH2 : invH2 * invH;
-    const T mF = J.m * k.cnorm * g * invHD1 * invr;           // m_j F
+    const T r = r2 * invr;
+    const T gH = k.grad_mode ? T(0.5) * (spline_gH<D>(r, fast_rcp(T(2) * I.h)) + spline_gH<D>(r, fast_rcp(T(2) * J.h)))
+                             : spline_gH<D>(r, fast_rcp(I.h + J.h));
+    const T mF = J.m * k.cnorm * gH * invr;                   // m_j F
     const T pt = I.pr + J.pr + Pi;
     const T vc = mF * T(2) * (I.nu + J.nu) * r2 * inv_rs * fast_rcp(den);
     const T cp = -mF * pt;

@@ -95,6 +95,8 @@ void check_pass_args(const wcsph_params* p, const wcsph_pass_desc* d, const doub
     WCSPH_REQUIRE(p->rho0 > 0 && p->c0 > 0 && p->gamma > 0 && p->c_cfl > 0 && p->eps >= 0, WCSPH_EINVAL,
                   "rho0, c0, gamma, c_cfl must be positive and eps non-negative");
     WCSPH_REQUIRE(d->n > 0 && d->n < ((int64_t)1 << 27), WCSPH_EINVAL, "n must be in [1, 2^27)");
+    WCSPH_REQUIRE(p->grad_mode == 0, WCSPH_EINVAL,
+                  "the force pass takes the gradient at the mean smoothing length only (grad_mode 0)");
     WCSPH_REQUIRE(pos && vel && dens && mass && h && acc && div_v && dt_i, WCSPH_EINVAL, "null array");
 }
 
@@ -183,6 +185,7 @@ void run_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const PassG
     a.k.eps = (float)p->eps;
     a.k.cnorm = (float)(D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI);  // src/kernel.h:35-41
     a.k.vsig_mode = p->vsig_mode;
+    a.k.grad_mode = 0;
     a.acc = acc;
     a.divv = div_v;
     a.dti = dt_i;

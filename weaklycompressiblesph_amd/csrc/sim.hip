@@ -105,6 +105,7 @@ void check_params(const wcsph_params* p) {
     WCSPH_REQUIRE(p->rho0 > 0 && p->c0 > 0 && p->gamma > 0 && p->c_cfl > 0 && p->eps >= 0, WCSPH_EINVAL,
                   "rho0, c0, gamma, c_cfl must be positive and eps non-negative");
     WCSPH_REQUIRE(p->shepard_interval >= 0, WCSPH_EINVAL, "shepard_interval must be non-negative");
+    WCSPH_REQUIRE(p->grad_mode == 0 || p->grad_mode == 1, WCSPH_EINVAL, "grad_mode must be 0 or 1");
 }
 
 constexpr int kDiag = 10;  // diagnostic sums
@@ -257,6 +258,10 @@ static void scan_h(wcsph_sim* s, const double* h) {
     }
     s->h_const = same ? h[0] : 0.0;
     WCSPH_REQUIRE(hmax > 0.0, WCSPH_EINVAL, "smoothing lengths must be positive");
+    // with one smoothing length both gradient forms are the same pair body
+    WCSPH_REQUIRE(s->p.grad_mode == 0 || same, WCSPH_EINVAL,
+                  "grad_mode 1 (the mean of the gradients at h_i and h_j) needs one smoothing length for all "
+                  "particles on the device-resident paths (the FDPS drop-in takes it with any h)");
     s->hmax = (float)hmax;
 }
 
@@ -408,6 +413,7 @@ static PairConsts<T> pair_consts(const wcsph_sim* s) {
     k.eps = (T)s->p.eps;
     k.cnorm = (T)(s->D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI);
     k.vsig_mode = s->p.vsig_mode;
+    k.grad_mode = 0;  // grad_mode 1 is taken only with one smoothing length, where both forms agree
     return k;
 }
 
@@ -1177,7 +1183,7 @@ int wcsph_sim_diagnostics(wcsph_sim* s, double zfront, double xcol, double xwall
 namespace {
 
 constexpr char kRestartMagic[8] = {'W', 'C', 'S', 'P', 'H', 'R', 'S', 'T'};
-constexpr uint32_t kRestartVersion = 1;
+constexpr uint32_t kRestartVersion = 2;
 
 // Restart file header; the payload follows it (see restart_arrays).
 struct RestartHeader {
