@@ -1050,7 +1050,8 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
             float hj[kScanUnroll];
 #pragma unroll
             for (int u = 0; u < kScanUnroll; ++u) {
-                const uint32_t j = p0 + min(t + u, len ? len - 1 : 0u);
+                // an empty window may start at n (cells past the last particle): read record 0
+                const uint32_t j = len ? p0 + min(t + u, len - 1) : 0u;
                 pj[u] = a.pk.px[j];
                 hj[u] = UH ? I.h : a.pk.pq[j].w;
             }
@@ -1162,54 +1163,98 @@ __global__ __launch_bounds__(kBlock) void k_shepard(Master m, Packed pk, const u
 //   T = sum m_j W_ij q_j,  so  rho_i' = rho0 / (1 + T / S),
 // and FP32 rounding of S and T moves T / S, and rho' - rho0, only by parts
 // in 1e7. W's normalisation is common to both sums and left out.
+//
+// The scan is the force kernels': the wave walks the (2R+1)^(D-1) rows of
+// cells around its particles in lockstep, each lane clipping its own x
+// window to the chord of its support sphere (2 h_i, the filter's support,
+// src/force.h:20-26), and every candidate in the window is evaluated in
+// place (the body is short, so no queue). Self is included (r = 0).
 template <int D, int R>
 __global__ __launch_bounds__(kBlock) void k_shepard32(Packed pk, const uint32_t* __restrict__ start, Grid g,
                                                       int64_t n, double rho0, const double* __restrict__ dens,
                                                       double* __restrict__ out, const Scalars* sc) {
+    constexpr int S = 2 * R + 1;
+    constexpr int NR = D == 3 ? S * S : S;
+    constexpr int U = 4;  // candidates loaded per trip
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (sim_failed(sc)) {  // a diverged state keeps its densities
-        out[i] = dens[i];
-        return;
-    }
-    const float4 xi4 = pk.px[i];
+    const bool failed = sim_failed(sc);
+    const bool live = i < n && !failed;
+    const int64_t ii = live ? i : 0;
+    const float4 xi4 = pk.px[ii];
     const float xi[3] = {xi4.x, xi4.y, xi4.z};
-    const float H = 2.f * pk.pq[i].w, H2 = H * H, invH = 1.f / H;
-    int c[3];
-    key_cell<D>(g, pk.pcell[i], c);
-    const float hc = H * 1.0001f;
-    Stencil<D, R> st;
-    st.k = -1;
-    st.advance(g, start, c, xi, hc * hc);
-    float S = 0.f, T = 0.f;
-    while (st.k < Stencil<D, R>::NS) {
-        float dc[3];
-#pragma unroll
-        for (int d = 0; d < D; ++d) dc[d] = Stencil<D, R>::dcell(st.k, d);
-        for (uint32_t j = st.p; j < st.pend; ++j) {
-            const float4 pj = pk.px[j];
-            const float pjv[3] = {pj.x, pj.y, pj.z};
-            float r2 = 0.f;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const float dd = fmaf(dc[d], g.Lf, xi[d]) - pjv[d];
-                r2 += dd * dd;
-            }
-            if (r2 < H2) {
-                const float s = sqrtf(r2) * invH;
-                const float a = 1.f - s, b = 0.5f - s;
-                float w = a * a * a;
-                if (b > 0.f) w -= 4.f * b * b * b;
-                const double rj = dens[j];
-                const float q = (float)(rho0 - rj) / (float)rj;
-                const float mw = pk.pv[j].w * w;
-                S += mw;
-                T += mw * q;
+    const float cxi = xi4.w;
+    const float H = 2.f * pk.pq[ii].w, H2 = H * H, invH = 1.f / H;
+    const float Lf = g.Lf, invLf = 1.f / Lf;
+    const float hc = H * 1.0001f, hc2 = hc * hc;
+    int c[3] = {0, 0, 0};
+    key_cell<D>(g, pk.pcell[ii], c);
+    // candidate range of row r for this lane: indices [start[b0], start[b1])
+    auto row_bounds = [&](int r, uint32_t& b0, uint32_t& b1) {
+        const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
+        const int cy = c[1] + dy, cz = c[2] + dz;
+        b0 = b1 = 0;
+        if (live && cy >= 0 && cy < g.nc[1] && (D == 2 || (cz >= 0 && cz < g.nc[2]))) {
+            const float gy = axis_gap(dy, xi[1], Lf, cy, g.nc[1]);
+            const float gz = D == 3 ? axis_gap(dz, xi[2], Lf, cz, g.nc[2]) : 0.f;
+            const float g2 = gy * gy + gz * gz;
+            if (g2 < hc2) {
+                const float w = sqrtf(hc2 - g2);
+                int xa = (int)floorf((xi[0] - w) * invLf), xb = (int)floorf((xi[0] + w) * invLf);
+                // the row's end cells are open outward (axis_gap)
+                const int kmin = -c[0], kmax = g.nc[0] - 1 - c[0];
+                xa = min(max(max(xa, -R), kmin), kmax);
+                xb = max(min(min(xb, R), kmax), kmin);
+                if (xa <= xb) {
+                    const uint32_t row = D == 3 ? (uint32_t)((cz * g.nc[1] + cy) * g.nc[0]) : (uint32_t)(cy * g.nc[0]);
+                    b0 = row + c[0] + xa;
+                    b1 = row + c[0] + xb + 1;
+                }
             }
         }
-        st.advance(g, start, c, xi, hc * hc);
+    };
+    float Ssum = 0.f, Tsum = 0.f;
+    uint32_t nb0, nb1;
+    row_bounds(0, nb0, nb1);
+    uint32_t np0 = nb1 ? start[nb0] : 0u, npe = nb1 ? start[nb1] : 0u;
+    for (int r = 0; r < NR; ++r) {
+        const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
+        const uint32_t p0 = np0, len = npe - np0;
+        if (r + 1 < NR) {  // the next row's range, fetched behind this row's scan
+            row_bounds(r + 1, nb0, nb1);
+            np0 = nb1 ? start[nb0] : 0u;
+            npe = nb1 ? start[nb1] : 0u;
+        }
+        const float yi = fmaf((float)(-dy), Lf, xi[1]);
+        const float zi = D == 3 ? fmaf((float)(-dz), Lf, xi[2]) : 0.f;
+        for (uint32_t t = 0; __any(t < len); t += U) {
+            float4 pj[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) pj[u] = pk.px[len ? p0 + min(t + u, len - 1) : 0u];  // in bounds when empty
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float dx = fmaf(cxi - pj[u].w, Lf, xi[0]) - pj[u].x;
+                const float dyv = yi - pj[u].y;
+                float r2 = dx * dx + dyv * dyv;
+                if (D == 3) {
+                    const float dzv = zi - pj[u].z;
+                    r2 += dzv * dzv;
+                }
+                if (t + u < len && r2 < H2) {
+                    const uint32_t j = p0 + t + u;
+                    const float s = sqrtf(r2) * invH;
+                    const float a = 1.f - s, b = 0.5f - s;
+                    float w = a * a * a;
+                    if (b > 0.f) w -= 4.f * b * b * b;
+                    const double rj = dens[j];
+                    const float q = (float)(rho0 - rj) / (float)rj;
+                    const float mw = pk.pv[j].w * w;
+                    Ssum += mw;
+                    Tsum += mw * q;
+                }
+            }
+        }
     }
-    out[i] = rho0 / (1.0 + (double)T / (double)S);
+    if (i < n) out[i] = failed ? dens[i] : rho0 / (1.0 + (double)Tsum / (double)Ssum);  // a diverged state keeps its densities
 }
 
 // Commit the filtered density (copyFromForce, src/class.h:116-118) and redo
