@@ -597,43 +597,6 @@ __device__ __forceinline__ float axis_gap(int d, float x, float Lf, int t, int n
     return fmaxf(fmaxf(lo - x, x - hi), 0.f);
 }
 
-template <int D, int R>
-struct Stencil {
-    static constexpr int S = 2 * R + 1;
-    static constexpr int NS = D == 3 ? S * S * S : S * S;
-    int k;
-    uint32_t p, pend;
-
-    __device__ __forceinline__ void advance(const Grid& g, const uint32_t* __restrict__ start, const int* c,
-                                            const float* xl, float hcut2) {
-        while (++k < NS) {
-            const int d0 = k % S - R, d1 = (k / S) % S - R, d2 = D == 3 ? k / (S * S) - R : 0;
-            const int c0 = c[0] + d0, c1 = c[1] + d1, c2 = c[2] + d2;
-            if (c0 < 0 || c0 >= g.nc[0] || c1 < 0 || c1 >= g.nc[1]) continue;
-            if (D == 3 && (c2 < 0 || c2 >= g.nc[2])) continue;
-            if (R > 1) {
-                const int dd[3] = {d0, d1, d2};
-                float gap2 = 0.f;
-#pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    const float gp = axis_gap(dd[a], xl[a], g.Lf, c[a] + dd[a], g.nc[a]);
-                    gap2 += gp * gp;
-                }
-                if (gap2 > hcut2) continue;
-            }
-            const uint32_t cell = D == 3 ? (uint32_t)((c2 * g.nc[1] + c1) * g.nc[0] + c0)
-                                         : (uint32_t)(c1 * g.nc[0] + c0);
-            p = start[cell];
-            pend = start[cell + 1];
-            if (p < pend) return;
-        }
-    }
-    // (c_i - c_j) for the cell of stencil slot kk, per axis, as a float
-    __device__ __forceinline__ static float dcell(int kk, int axis) {
-        const int d = axis == 0 ? kk % S - R : axis == 1 ? (kk / S) % S - R : kk / (S * S) - R;
-        return (float)(-d);
-    }
-};
 
 // The neighbour predicate on cell-local coordinates: per axis
 // d = fma(c_i - c_j, L, xl_i) - xl_j, r2 = fma(dz, dz, fma(dy, dy, dx dx)),
@@ -789,37 +752,97 @@ __device__ __forceinline__ void block_count_atomic(uint32_t cnt, Scalars* s, boo
 // Neighbour lists (FP64 path and on-demand list download): every j with
 // the cell-local FP32 predicate, in stencil order, slot t of all i stored
 // contiguously.
+// The candidate windows of one particle in the lockstep row scan: row r of
+// the (2R+1)^(D-1) rows of cells around cell c (z-major, then y) gives the
+// cells [b0, b1) of that row that the support chord of radius hc reaches
+// (b0 = b1 = 0 when none); the rows and the cells within a row come in the
+// order of a z, y, x walk over the stencil. The row's end cells are open
+// outward (axis_gap), so a chord reaching past one keeps it.
+template <int D, int R>
+struct RowWindows {
+    static constexpr int S = 2 * R + 1;
+    static constexpr int NR = D == 3 ? S * S : S;
+    const Grid& g;
+    const int* c;
+    const float* xi;
+    float hc2, invLf;
+    bool live;
+
+    __device__ __forceinline__ void bounds(int r, uint32_t& b0, uint32_t& b1) const {
+        const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
+        const int cy = c[1] + dy, cz = c[2] + dz;
+        b0 = b1 = 0;
+        if (live && cy >= 0 && cy < g.nc[1] && (D == 2 || (cz >= 0 && cz < g.nc[2]))) {
+            const float gy = axis_gap(dy, xi[1], g.Lf, cy, g.nc[1]);
+            const float gz = D == 3 ? axis_gap(dz, xi[2], g.Lf, cz, g.nc[2]) : 0.f;
+            const float g2 = gy * gy + gz * gz;
+            if (g2 < hc2) {
+                const float w = sqrtf(hc2 - g2);
+                int xa = (int)floorf((xi[0] - w) * invLf), xb = (int)floorf((xi[0] + w) * invLf);
+                const int kmin = -c[0], kmax = g.nc[0] - 1 - c[0];
+                xa = min(max(max(xa, -R), kmin), kmax);
+                xb = max(min(min(xb, R), kmax), kmin);
+                if (xa <= xb) {
+                    const uint32_t row = D == 3 ? (uint32_t)((cz * g.nc[1] + cy) * g.nc[0]) : (uint32_t)(cy * g.nc[0]);
+                    b0 = row + c[0] + xa;
+                    b1 = row + c[0] + xb + 1;
+                }
+            }
+        }
+    }
+};
+
 template <int D, int R>
 __global__ __launch_bounds__(kBlock) void k_neigh(Packed pk, const uint32_t* __restrict__ start, Grid g,
                                                   int64_t n, int K, float hmax, uint32_t* __restrict__ nbr,
                                                   uint32_t* __restrict__ cnt, Partials P, const Scalars* s) {
+    // the lockstep row scan (RowWindows); lists come out in the order of a
+    // z, y, x walk over the stencil's cells, index order within a cell
+    constexpr int U = 4;  // candidates loaded per trip
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+    const bool failed = sim_failed(s);
+    const bool live = i < n && !failed;
+    const int64_t ii = live ? i : 0;
+    const float4 xi4 = pk.px[ii];
+    const float xi[3] = {xi4.x, xi4.y, xi4.z};
+    const float cxi = xi4.w;
+    const float hi = pk.pq[ii].w;
+    int c[3] = {0, 0, 0};
+    key_cell<D>(g, pk.pcell[ii], c);
+    const float hc = (hi + hmax) * 1.0001f;
+    const RowWindows<D, R> rw{g, c, xi, hc * hc, 1.f / g.Lf, live};
     uint32_t c_out = 0;
-    if (i < n && sim_failed(s)) cnt[i] = 0;
-    if (i < n && !sim_failed(s)) {
-        const float4 xi4 = pk.px[i];
-        const float xi[3] = {xi4.x, xi4.y, xi4.z};
-        const float hi = pk.pq[i].w;
-        int c[3];
-        key_cell<D>(g, pk.pcell[i], c);
-        const float hc = (hi + hmax) * 1.0001f;
-        Stencil<D, R> st;
-        st.k = -1;
-        st.advance(g, start, c, xi, hc * hc);
-        while (st.k < Stencil<D, R>::NS) {
-            float dc[3];
+    uint32_t nb0, nb1;
+    rw.bounds(0, nb0, nb1);
+    uint32_t np0 = nb1 ? start[nb0] : 0u, npe = nb1 ? start[nb1] : 0u;
+    for (int r = 0; r < RowWindows<D, R>::NR; ++r) {
+        const int dy = r % RowWindows<D, R>::S - R, dz = D == 3 ? r / RowWindows<D, R>::S - R : 0;
+        const uint32_t p0 = np0, len = npe - np0;
+        if (r + 1 < RowWindows<D, R>::NR) {  // the next row's range, fetched behind this row's scan
+            rw.bounds(r + 1, nb0, nb1);
+            np0 = nb1 ? start[nb0] : 0u;
+            npe = nb1 ? start[nb1] : 0u;
+        }
+        for (uint32_t t = 0; __any(t < len); t += U) {
+            float4 pj[U];
+            float hj[U];
 #pragma unroll
-            for (int d = 0; d < D; ++d) dc[d] = Stencil<D, R>::dcell(st.k, d);
-            for (uint32_t j = st.p; j < st.pend; ++j) {
-                if (nbr_pred<D>(xi, hi, pk.px[j], dc, g.Lf, pk.pq[j].w)) {
-                    if ((int)c_out < K) nbr[(int64_t)c_out * n + i] = j;
+            for (int u = 0; u < U; ++u) {
+                const uint32_t j = len ? p0 + min(t + u, len - 1) : 0u;  // in bounds when empty
+                pj[u] = pk.px[j];
+                hj[u] = pk.pq[j].w;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float dc[3] = {cxi - pj[u].w, (float)(-dy), (float)(-dz)};
+                if (t + u < len && nbr_pred<D>(xi, hi, pj[u], dc, g.Lf, hj[u])) {
+                    if ((int)c_out < K) nbr[(int64_t)c_out * n + i] = p0 + t + u;
                     ++c_out;
                 }
             }
-            st.advance(g, start, c, xi, hc * hc);
         }
-        cnt[i] = c_out;
     }
+    if (i < n) cnt[i] = c_out;
     block_count_partial(c_out, c_out > (uint32_t)K ? 1u : 0u, P, blockIdx.x);
 }
 
@@ -1103,16 +1126,18 @@ template <int D, int R>
 __global__ __launch_bounds__(kBlock) void k_shepard(Master m, Packed pk, const uint32_t* __restrict__ start, Grid g,
                                                     int64_t n, double cnorm, double* __restrict__ out,
                                                     const Scalars* sc) {
+    // the lockstep row scan (RowWindows) over the cells of i's support;
+    // candidates come in the order of a z, y, x walk over the stencil's
+    // cells, and the FP64 predicate and sums are the oracle's
+    constexpr int U = 4;  // candidates loaded per trip
     const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (sim_failed(sc)) {  // a diverged state keeps its densities
-        out[i] = m.dens[i];
-        return;
-    }
-    double xi[D];
+    const bool failed = sim_failed(sc);
+    const bool live = i < n && !failed;
+    const int64_t ii = live ? i : 0;
+    double xi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int d = 0; d < D; ++d) xi[d] = m.pos[d * n + i];
-    const double H = 2.0 * m.h[i], H2 = H * H, invH = 1.0 / H;
+    for (int d = 0; d < D; ++d) xi[d] = m.pos[d * n + ii];
+    const double H = 2.0 * m.h[ii], H2 = H * H, invH = 1.0 / H;
     double norm = cnorm;  // C_D / H^D (src/kernel.h:42-48)
 #pragma unroll
     for (int d = 0; d < D; ++d) norm *= invH;
@@ -1125,32 +1150,49 @@ __global__ __launch_bounds__(kBlock) void k_shepard(Master m, Packed pk, const u
 #pragma unroll
     for (int d = 0; d < D; ++d) xl[d] = cell_local(xi[d], g.lo[d], c[d], g.L);
     const float hc = (float)H * 1.0001f;
-    Stencil<D, R> st;
-    st.k = -1;
-    st.advance(g, start, c, xl, hc * hc);
+    const RowWindows<D, R> rw{g, c, xl, hc * hc, 1.f / g.Lf, live};
     double one = 0.0, sum = 0.0;
-    while (st.k < Stencil<D, R>::NS) {
-        for (uint32_t j = st.p; j < st.pend; ++j) {
-            double r2 = 0.0;
+    uint32_t nb0, nb1;
+    rw.bounds(0, nb0, nb1);
+    uint32_t np0 = nb1 ? start[nb0] : 0u, npe = nb1 ? start[nb1] : 0u;
+    for (int r = 0; r < RowWindows<D, R>::NR; ++r) {
+        const uint32_t p0 = np0, len = npe - np0;
+        if (r + 1 < RowWindows<D, R>::NR) {  // the next row's range, fetched behind this row's scan
+            rw.bounds(r + 1, nb0, nb1);
+            np0 = nb1 ? start[nb0] : 0u;
+            npe = nb1 ? start[nb1] : 0u;
+        }
+        for (uint32_t t = 0; __any(t < len); t += U) {
+            double xj[U][3];
 #pragma unroll
-            for (int d = 0; d < D; ++d) {
-                const double dd = xi[d] - m.pos[d * n + j];
-                r2 += dd * dd;
+            for (int u = 0; u < U; ++u) {
+                const uint32_t j = len ? p0 + min(t + u, len - 1) : 0u;  // in bounds when empty
+#pragma unroll
+                for (int d = 0; d < D; ++d) xj[u][d] = m.pos[d * n + j];
             }
-            if (r2 < H2) {
-                const double s = sqrt(r2) * invH;
-                const double a = 1.0 - s, b = 0.5 - s;
-                double w = a * a * a;
-                if (b > 0.0) w -= 4.0 * b * b * b;
-                w *= norm;
-                const double mj = m.mass[j];
-                one += w * mj / m.dens[j];
-                sum += mj * w;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                double r2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    const double dd = xi[d] - xj[u][d];
+                    r2 += dd * dd;
+                }
+                if (t + u < len && r2 < H2) {
+                    const uint32_t j = p0 + t + u;
+                    const double s = sqrt(r2) * invH;
+                    const double a = 1.0 - s, b = 0.5 - s;
+                    double w = a * a * a;
+                    if (b > 0.0) w -= 4.0 * b * b * b;
+                    w *= norm;
+                    const double mj = m.mass[j];
+                    one += w * mj / m.dens[j];
+                    sum += mj * w;
+                }
             }
         }
-        st.advance(g, start, c, xl, hc * hc);
     }
-    out[i] = sum / one;
+    if (i < n) out[i] = failed ? m.dens[i] : sum / one;  // a diverged state keeps its densities
 }
 
 // The same filter in FP32 arithmetic for the FP32 and mixed paths, from the
@@ -1184,43 +1226,20 @@ __global__ __launch_bounds__(kBlock) void k_shepard32(Packed pk, const uint32_t*
     const float xi[3] = {xi4.x, xi4.y, xi4.z};
     const float cxi = xi4.w;
     const float H = 2.f * pk.pq[ii].w, H2 = H * H, invH = 1.f / H;
-    const float Lf = g.Lf, invLf = 1.f / Lf;
-    const float hc = H * 1.0001f, hc2 = hc * hc;
+    const float Lf = g.Lf;
+    const float hc = H * 1.0001f;
     int c[3] = {0, 0, 0};
     key_cell<D>(g, pk.pcell[ii], c);
-    // candidate range of row r for this lane: indices [start[b0], start[b1])
-    auto row_bounds = [&](int r, uint32_t& b0, uint32_t& b1) {
-        const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
-        const int cy = c[1] + dy, cz = c[2] + dz;
-        b0 = b1 = 0;
-        if (live && cy >= 0 && cy < g.nc[1] && (D == 2 || (cz >= 0 && cz < g.nc[2]))) {
-            const float gy = axis_gap(dy, xi[1], Lf, cy, g.nc[1]);
-            const float gz = D == 3 ? axis_gap(dz, xi[2], Lf, cz, g.nc[2]) : 0.f;
-            const float g2 = gy * gy + gz * gz;
-            if (g2 < hc2) {
-                const float w = sqrtf(hc2 - g2);
-                int xa = (int)floorf((xi[0] - w) * invLf), xb = (int)floorf((xi[0] + w) * invLf);
-                // the row's end cells are open outward (axis_gap)
-                const int kmin = -c[0], kmax = g.nc[0] - 1 - c[0];
-                xa = min(max(max(xa, -R), kmin), kmax);
-                xb = max(min(min(xb, R), kmax), kmin);
-                if (xa <= xb) {
-                    const uint32_t row = D == 3 ? (uint32_t)((cz * g.nc[1] + cy) * g.nc[0]) : (uint32_t)(cy * g.nc[0]);
-                    b0 = row + c[0] + xa;
-                    b1 = row + c[0] + xb + 1;
-                }
-            }
-        }
-    };
+    const RowWindows<D, R> rw{g, c, xi, hc * hc, 1.f / Lf, live};
     float Ssum = 0.f, Tsum = 0.f;
     uint32_t nb0, nb1;
-    row_bounds(0, nb0, nb1);
+    rw.bounds(0, nb0, nb1);
     uint32_t np0 = nb1 ? start[nb0] : 0u, npe = nb1 ? start[nb1] : 0u;
     for (int r = 0; r < NR; ++r) {
         const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
         const uint32_t p0 = np0, len = npe - np0;
         if (r + 1 < NR) {  // the next row's range, fetched behind this row's scan
-            row_bounds(r + 1, nb0, nb1);
+            rw.bounds(r + 1, nb0, nb1);
             np0 = nb1 ? start[nb0] : 0u;
             npe = nb1 ? start[nb1] : 0u;
         }
