@@ -658,8 +658,10 @@ def profiled_traffic(kernel):
     vals = {}
     for line in open(path):
         f = line.split()
-        if len(f) >= 4 and f[0].startswith(kernel + "<") and f[-4] in ("FETCH_SIZE", "WRITE_SIZE"):
-            vals[f[-4]] = float(f[-2])
+        if len(f) >= 4 and f[0].startswith(kernel + "<"):
+            for k in ("FETCH_SIZE", "WRITE_SIZE"):
+                if k in f and f.index(k) + 2 < len(f):  # counter, dispatches, mean per dispatch
+                    vals[k] = float(f[f.index(k) + 2])
     if len(vals) < 2:
         return None
     b = (2.0 * vals["FETCH_SIZE"] + vals["WRITE_SIZE"]) * 1024.0
