@@ -117,6 +117,19 @@ def test_grouped_steps_equal_simulation(W, dim):
     assert res["front"] == d["front"] and res["column"] == d["column"] and res["kinetic"] == d["kinetic"]
 
 
+def test_serving_single_step_chunks(W):
+    # chunk_steps 1: every grouped member advances by one-member single
+    # steps, and the results equal those of two-step graph chunks bit for bit
+    spec = [(2, 20_000, 1.0, 0.02, 11), (2, 30_000, 2.0, 0.05, 6), (3, 25_000, 1.5, 0.12, 7)]
+    reqs = [W.make_request(d, n, a, al, s, 0.0, "mixed") for d, n, a, al, s in spec]
+    one = W.serve(reqs, max_slots=3, chunk_steps=1)
+    big = W.serve(reqs, max_slots=3, chunk_steps=25)
+    for k, r in enumerate(reqs):
+        assert one[k]["status"] == 0 and one[k]["steps"] == r.steps
+        for key in KEYS:
+            assert one[k][key] == big[k][key], (k, key)
+
+
 def test_serving_rejects_oversized_request(W):
     reqs = small_stream(W)[:1]
     with pytest.raises(W.WcsphError, match="budget"):

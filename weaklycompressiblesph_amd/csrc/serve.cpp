@@ -259,8 +259,10 @@ extern "C" int wcsph_serve(const wcsph_serve_config* cfg, const wcsph_request* r
                     for (Job& J : active)
                         if (J.group == g) J.done_steps += chunk / 2 * 2;
                 } else {
+                    // a chunk of one step (chunk_steps 1, or a member one step
+                    // from its end): every member takes one single step
                     for (Job& J : active)
-                        if (J.group == g && reqs[J.idx].steps - J.done_steps == 1) {
+                        if (J.group == g) {
                             group_run_single(R.groups[g], J.sim);
                             J.done_steps += 1;
                         }
