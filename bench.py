@@ -43,7 +43,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOP_PER_PAIR = {3: 73, 2: 63}       # algorithmic flops per pair interaction
-PEAK_VALU_TFLOPS = 157.3             # MI355X published FP32 (packed) / packed-FP16 vector peak
+PEAK_VALU_TFLOPS = 157.3             # MI355X published FP32 / packed-FP16 vector peak
 PEAK_HBM_GBS = 8000.0
 
 
@@ -561,9 +561,11 @@ def main():
             if "valu_issue_frac" in prof:
                 line["roofline"]["valu_issue_frac"] = prof["valu_issue_frac"]
     line["hbm_copy_GBps"] = W.copy_probe_gbps(1 << 30, 10)
-    # measured VALU FMA rates: the attainable peaks of the two instruction kinds the force passes use
+    # measured VALU FMA rates: the attainable peaks of the instruction kinds the force passes use (FP32
+    # FMA with VGPR operands, packed FP16 FMA) and FP32 FMA with a wave-uniform SGPR operand, which gfx950
+    # issues at half the rate of the VGPR form
     peaks = {"fp32_fma": W.valu_probe_tflops("fp32"), "pk_f16_fma": W.valu_probe_tflops("pk_f16")}
-    line["valu_probe_TFLOPs"] = peaks
+    line["valu_probe_TFLOPs"] = dict(peaks, fp32_fma_sgpr_operand=W.valu_probe_tflops("fp32_sgpr"))
     line["roofline"]["frac_of_measured"] = {k: achieved / v for k, v in peaks.items()}
     if rank == 0 and args.cpu_seconds > 0:
         # the box's CPU share is 16 threads per GPU, whatever nproc shows

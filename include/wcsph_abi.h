@@ -93,9 +93,11 @@ const char* wcsph_version(void);
 int wcsph_copy_probe(int32_t device, int64_t bytes, int32_t reps, double* gbps);
 /* VALU rate probe: independent FMA chains in every lane of a full grid on
  * `device`, best of `reps` launches; *tflops = 2 x FMAs / time. kind 0: FP32
- * FMA (v_fma_f32), kind 1: packed FP16 FMA (v_pk_fma_f16, two FMAs per
- * instruction). Used by the bench for the measured VALU peaks (kernel
- * k_valu). Neither probe has a counterpart in the reference. */
+ * FMA (v_fma_f32) reading only VGPRs and constants, kind 1: packed FP16 FMA
+ * (v_pk_fma_f16, two FMAs per instruction), kind 2: FP32 FMA with one
+ * wave-uniform SGPR operand (on gfx950 it issues at half the rate of kind
+ * 0). Used by the bench for the measured VALU peaks (kernel k_valu). Neither
+ * probe has a counterpart in the reference. */
 int wcsph_valu_probe(int32_t device, int32_t kind, int32_t reps, double* tflops);
 
 /* ---------------- 1. FDPS multiwalk drop-in ---------------- */

@@ -116,9 +116,10 @@ def copy_probe_gbps(nbytes=1 << 30, reps=10, device=0):
 
 
 def valu_probe_tflops(kind, reps=5, device=0):
-    """Measured VALU FMA rate in TFLOP/s: kind "fp32" (v_fma_f32) or
-    "pk_f16" (v_pk_fma_f16)."""
-    k = {"fp32": 0, "pk_f16": 1}[kind]
+    """Measured VALU FMA rate in TFLOP/s: kind "fp32" (v_fma_f32 with VGPR
+    and constant operands), "pk_f16" (v_pk_fma_f16) or "fp32_sgpr"
+    (v_fma_f32 with one wave-uniform SGPR operand)."""
+    k = {"fp32": 0, "pk_f16": 1, "fp32_sgpr": 2}[kind]
     v = C.c_double()
     check(lib().wcsph_valu_probe(device, k, reps, C.byref(v)))
     return v.value

@@ -2,9 +2,12 @@
 // - HBM copy probe: a plain streaming copy of known size, used to measure the
 //   attainable HBM bandwidth and to calibrate hardware byte counters.
 // - VALU probe: independent FMA chains in every lane of a full grid, timed
-//   with HIP events, giving the attainable rate of FP32 FMA (v_fma_f32; this
-//   build disables packed FP32) and of packed FP16 FMA (v_pk_fma_f16), the
-//   two instruction kinds the force passes are built from.
+//   with HIP events. Kind 0: FP32 FMA (v_fma_f32) with VGPR and constant
+//   operands only; kind 1: packed FP16 FMA (v_pk_fma_f16, two FMAs per
+//   instruction); kind 2: FP32 FMA with one wave-uniform (SGPR) operand.
+//   On gfx950 kind 0 issues a wave64 instruction every 2 cycles and kinds 1
+//   and 2 every 4, so the FP32 rate depends on the operand form, and packed
+//   FP16 reaches the same flop rate as FP32 FMA with VGPR operands.
 #include "common.hpp"
 
 namespace {
@@ -26,26 +29,33 @@ struct ProbeResources {
 };
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-constexpr int kValuChains = 8;  // independent chains per lane, enough to cover the FMA latency
+constexpr int kValuChains = 12;  // independent chains per lane, enough to cover the FMA latency
 
-// KIND 0: x = x a + b in FP32; KIND 1: the same on half2 (two FMAs per op).
-// The chains converge to b / (1 - a), so no value overflows or goes subnormal.
+// KIND 0: x = x a + 1/2 in FP32 with a in a VGPR (a per-lane copy), so the
+// FMA reads only VGPRs and an inline constant; KIND 2: the same with a
+// read from its SGPR; KIND 1: the packed FP16 form (two FMAs per op). The
+// chains converge to 0.5 / (1 - a), so no value overflows or goes
+// subnormal.
 template <int KIND>
 __global__ __launch_bounds__(256) void k_valu(float* __restrict__ out, int32_t iters, float a, float b) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // a per-lane value the compiler keeps in a VGPR (it differs from a by
+    // less than one rounding)
+    const float av = a + b * 1.0e-12f * (float)(threadIdx.x & 7);
     float sum = 0.f;
-    if (KIND == 0) {
+    if (KIND != 1) {
+        const float m = KIND == 0 ? av : a;
         float x[kValuChains];
 #pragma unroll
         for (int k = 0; k < kValuChains; ++k) x[k] = (float)(threadIdx.x + k) * 1e-3f;
         for (int32_t it = 0; it < iters; ++it) {
 #pragma unroll
-            for (int k = 0; k < kValuChains; ++k) x[k] = __builtin_fmaf(x[k], a, b);
+            for (int k = 0; k < kValuChains; ++k) x[k] = __builtin_fmaf(x[k], m, 0.5f);
         }
 #pragma unroll
         for (int k = 0; k < kValuChains; ++k) sum += x[k];
     } else {
-        const half2_t a2 = {(_Float16)a, (_Float16)a}, b2 = {(_Float16)b, (_Float16)b};
+        const half2_t a2 = {(_Float16)av, (_Float16)av}, b2 = {(_Float16)0.5f, (_Float16)0.5f};
         half2_t x[kValuChains];
 #pragma unroll
         for (int k = 0; k < kValuChains; ++k) x[k] = half2_t{(_Float16)(threadIdx.x * 1e-3f), (_Float16)(k * 1e-3f)};
@@ -88,8 +98,9 @@ extern "C" int wcsph_copy_probe(int32_t device, int64_t bytes, int32_t reps, dou
 
 extern "C" int wcsph_valu_probe(int32_t device, int32_t kind, int32_t reps, double* tflops) {
     return wcsph::guarded([&] {
-        WCSPH_REQUIRE((kind == 0 || kind == 1) && reps > 0 && tflops, WCSPH_EINVAL,
-                      "kind 0 (FP32 FMA) or 1 (packed FP16 FMA), reps > 0, tflops non-null");
+        WCSPH_REQUIRE(kind >= 0 && kind <= 2 && reps > 0 && tflops, WCSPH_EINVAL,
+                      "kind 0 (FP32 FMA), 1 (packed FP16 FMA) or 2 (FP32 FMA, SGPR operand), reps > 0, "
+                      "tflops non-null");
         WCSPH_HIP(hipSetDevice(device));
         hipDeviceProp_t prop;
         WCSPH_HIP(hipGetDeviceProperties(&prop, device));
@@ -103,9 +114,11 @@ extern "C" int wcsph_valu_probe(int32_t device, int32_t kind, int32_t reps, doub
         hipEvent_t e0 = r.ev[0], e1 = r.ev[1];
         auto launch = [&] {
             if (kind == 0)
-                hipLaunchKernelGGL(k_valu<0>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1.0e-3f);
+                hipLaunchKernelGGL(k_valu<0>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1.0f);
+            else if (kind == 1)
+                hipLaunchKernelGGL(k_valu<1>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1.0f);
             else
-                hipLaunchKernelGGL(k_valu<1>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1.0e-3f);
+                hipLaunchKernelGGL(k_valu<2>, dim3(blocks), dim3(256), 0, 0, out, iters, 0.999f, 1.0f);
         };
         launch();  // warm-up
         float best = 1e30f;
