@@ -118,6 +118,7 @@ def _load(path):
     L.orc_sim_eos.argtypes = [C.POINTER(SimParams), C.POINTER(_CState)]
     L.orc_sim_force.restype = C.c_int64
     L.orc_sim_force.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), C.c_int]
+    L.orc_sim_force_scales.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), C.c_int, f64p, f64p]
     L.orc_global_dt.restype = C.c_double
     L.orc_global_dt.argtypes = [C.POINTER(SimParams), C.POINTER(_CState)]
     L.orc_conservation.argtypes = [C.POINTER(SimParams), C.POINTER(_CState), f64p]
@@ -305,6 +306,14 @@ class State:
 
     def force(self, p, nthreads=1):
         return lib().orc_sim_force(C.byref(p), C.byref(self.c), nthreads)
+
+    def force_scales(self, p, nthreads=1):
+        """Per-particle scales of the force pass's sums (the state is not
+        changed): sum_j |a_ij|_2 and sum_j |div v_ij| / rho_i over the pair
+        terms, each term alone. Per-particle errors are stated against them."""
+        sa, sd = np.zeros(self.n), np.zeros(self.n)
+        lib().orc_sim_force_scales(C.byref(p), C.byref(self.c), nthreads, sa, sd)
+        return sa, sd
 
     def global_dt(self, p):
         return lib().orc_global_dt(C.byref(p), C.byref(self.c))

@@ -527,7 +527,13 @@ static void state_pv(const orc_sim_params* p, const orc_state* s, int64_t i,
     o->snds = s->snds[i]; o->visc = p->nu; o->mass = s->mass[i];
 }
 
-int64_t orc_sim_force(const orc_sim_params* p, orc_state* s, int nthreads) {
+/* The force pass over all particles with a cell list. With acc_scale (and
+ * divv_scale) non-null it leaves the state alone and writes instead, per
+ * particle, the sum over j of the magnitudes of the pair terms
+ * (sum_j |a_ij|_2 and sum_j |div v_ij|, each term from pair_term alone):
+ * the scale against which the cancellation in the sums is judged. */
+static int64_t force_pass(const orc_sim_params* p, orc_state* s, int nthreads, double* acc_scale,
+                          double* divv_scale) {
     const int D = p->dim;
     const int64_t n = s->n;
     orc_force_params fp = {D, p->vsig_mode, p->alpha, p->beta, p->eps, p->c_cfl, 0};
@@ -583,16 +589,40 @@ int64_t orc_sim_force(const orc_sim_params* p, orc_state* s, int nthreads) {
                     for (int64_t t = cstart[cc]; t < cstart[cc + 1]; ++t) {
                         const int64_t j = cidx[t];
                         state_pv(p, s, j, pj, vj, &J);
-                        pair_term(&fp, D, &I, &J, acc, &divv, &vsig);
+                        if (acc_scale) {
+                            double ta[3] = {0, 0, 0}, td = 0, tv = 0;
+                            if (pair_term(&fp, D, &I, &J, ta, &td, &tv)) {
+                                double t2 = 0;
+                                for (int k = 0; k < D; ++k) t2 += ta[k] * ta[k];
+                                acc[0] += sqrt(t2);
+                                divv += fabs(td);
+                            }
+                        } else {
+                            pair_term(&fp, D, &I, &J, acc, &divv, &vsig);
+                        }
                         if (j != i && orc_nbr_pred(D, pi, pj, hf[i], hf[j])) ++pairs;
                     }
                 }
+        if (acc_scale) {
+            acc_scale[i] = acc[0];
+            divv_scale[i] = divv / I.dens;
+            continue;
+        }
         for (int k = 0; k < D; ++k) s->a[k * n + i] = acc[k];
         s->dt_i[i] = p->c_cfl * I.smth / (I.snds + vsig);
         s->divv[i] = divv / I.dens;
     }
     free(hf); free(cstart); free(cidx); free(ccell);
     return pairs;
+}
+
+int64_t orc_sim_force(const orc_sim_params* p, orc_state* s, int nthreads) {
+    return force_pass(p, s, nthreads, NULL, NULL);
+}
+
+void orc_sim_force_scales(const orc_sim_params* p, const orc_state* s, int nthreads, double* acc_scale,
+                          double* divv_scale) {
+    force_pass(p, (orc_state*)s, nthreads, acc_scale, divv_scale);
 }
 
 void orc_shepard(int dim, int64_t n, const double* x, const double* h, const double* mass,

@@ -142,6 +142,14 @@ int64_t orc_sim_step(const orc_sim_params* p, orc_state* s, int nthreads);
 double orc_global_dt(const orc_sim_params* p, const orc_state* s);
 /* force pass over all particles with a cell list; returns true pair count */
 int64_t orc_sim_force(const orc_sim_params* p, orc_state* s, int nthreads);
+/* The scale of each particle's sums in the same pass (no reference
+ * counterpart): acc_scale[i] = sum_j |a_ij|_2 and divv_scale[i] =
+ * sum_j |div v_ij| / rho_i over the pair terms of src/force.h:51-54, each
+ * term taken alone. Per-particle errors of a force pass are stated against
+ * these, because the sums cancel (at depth |a_i| is far below them). The
+ * state is not changed. */
+void orc_sim_force_scales(const orc_sim_params* p, const orc_state* s, int nthreads, double* acc_scale,
+                          double* divv_scale);
 
 /* diagnostics (no reference counterpart): out[0]=mass, out[1..3]=momentum,
  * out[4]=kinetic, out[5]=potential, out[6]=internal (Tait) energy */

@@ -373,6 +373,74 @@ def test_internal_energy_of_tait_law(O):
     assert e(1050.0) == pytest.approx(np.sum(0.5 * (f[1:] + f[:-1]) * np.diff(rr)), rel=1e-7)
 
 
+def _pair_state(O, dim, xs, dens, vel=None, h=0.013):
+    p = O.sim_params(dim, nu=1e-3)
+    s = O.State(dim, len(xs))
+    for i, x in enumerate(xs):
+        s.x[:, i] = x
+    s.dens[:] = dens
+    s.mass[:] = 1000.0 * 0.01 ** dim
+    s.h[:] = h
+    if vel is not None:
+        s.v[:] = np.asarray(vel, float).T
+    s.eos(p)
+    return p, s
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_force_scales_single_pair_is_the_magnitude(O, dim):
+    # with one neighbour each sum has one term: its scale is the magnitude
+    # of the force and of div v themselves (a scale that dropped a term,
+    # missed the absolute value or the 1/rho_i would differ)
+    x0 = np.zeros(dim)
+    x1 = np.array([0.011, 0.007, -0.004][:dim])
+    p, s = _pair_state(O, dim, [x0, x1], [1012.0, 1003.0], vel=[[0.3, -0.1, 0.2][:dim], [-0.2, 0.4, 0.1][:dim]])
+    s.force(p)
+    sa, sd = s.force_scales(p)
+    assert np.abs(s.a).max() > 0 and np.abs(s.divv).max() > 0
+    np.testing.assert_allclose(sa, np.linalg.norm(s.a, axis=0), rtol=1e-13)
+    np.testing.assert_allclose(sd, np.abs(s.divv), rtol=1e-13)
+    # the state is left as the force pass wrote it
+    a0 = s.a.copy()
+    s.force_scales(p)
+    assert np.array_equal(s.a, a0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_force_scales_count_cancelled_terms(O, dim):
+    # three particles on a line, equal densities, the ends beyond each
+    # other's support: the middle's two pair terms cancel exactly (a = 0),
+    # while its scale is the sum of their magnitudes, twice an end's
+    d = 0.02
+    xs = [np.eye(dim)[0] * k * d for k in (-1, 0, 1)]
+    p, s = _pair_state(O, dim, xs, [1010.0] * 3)
+    s.force(p)
+    sa, sd = s.force_scales(p)
+    end = np.linalg.norm(s.a[:, 0])
+    assert end > 0
+    assert np.linalg.norm(s.a[:, 1]) < 1e-12 * end
+    assert sa[1] == pytest.approx(2 * end, rel=1e-12)
+    assert sa[0] == pytest.approx(end, rel=1e-12) and sa[2] == pytest.approx(end, rel=1e-12)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_force_scales_bound_the_sums(O, dim):
+    # on a developed dam break every |a_i| and |div v_i| is at most its
+    # scale (triangle inequality), and at depth far below it
+    pp = O.sim_params(dim, alpha=0.1, nu=1e-6)
+    s = O.State.dam_break(dim, 0.05 if dim == 2 else 0.1)
+    s.init(pp)
+    for _ in range(20):
+        s.step(pp)
+    s.force(pp)
+    sa, sd = s.force_scales(pp)
+    an = np.linalg.norm(s.a, axis=0)
+    assert np.all(an <= sa * (1 + 1e-12))
+    assert np.all(np.abs(s.divv) <= sd * (1 + 1e-12) + 1e-300)
+    fl = s.type == 0
+    assert np.median(an[fl] / sa[fl]) < 0.5
+
+
 # --------------------------------------------------------------------------
 # neighbour sets, FDPS-style walks
 # --------------------------------------------------------------------------
