@@ -167,8 +167,15 @@ struct NFMixedArgs {
     // wave-uniform constants (kept in scalar registers)
     h2 k_eps, k_Ls;
     float malpha, beta;  // -2 alpha, 2 beta
-    double c0, nu, c_cfl, inv_rho0;
+    double inv_rho0;
     float gm1;           // gamma - 1 of the Tait law (sound speed of i in the epilogue)
+    // epilogue constants, formed once on the host: the force and viscous
+    // scales c0^2 / h and 4 nu c0 / h^2 (FP64), -c0 / h, c0 and C_CFL h
+    // (FP32), the viscous time-step bound 0.125 h^2 / nu (FP64), and whether
+    // gamma = 7, where the sound speed is c0 (rho / rho0)^3
+    double ks, kv, t2;
+    float kdivv, c0f, cfl_h, hf;
+    int g7;
     double* acc;
     double* divv;
     double* dti;
@@ -464,20 +471,20 @@ __device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& 
     double cand = failed ? 0.0 : 1.0e30;  // a diverged state stops the clock
     if (live) {
         const int64_t n = a.n;
-        // FP32 density and its Tait sound speed, c^2 = max(c0^2 (rho/rho0)^(gamma-1), 1e-16)
-        // (src/EoS.h:86)
-        const float rho = (float)a.dens[i], r = (float)(a.dens[i] * a.inv_rho0);
-        const float cs = sqrtf(fmaxf((float)(a.c0 * a.c0) * powf(r, a.gm1), 1.0e-16f));
-        const double c0 = a.c0, h = a.h;
-        const double ks = c0 * c0 / h, kv = 4.0 * a.nu * c0 / (h * h);
+        // FP32 density ratio and its Tait sound speed,
+        // c = sqrt(max(c0^2 (rho/rho0)^(gamma-1), 1e-16)) (src/EoS.h:86)
+        const float r = (float)(a.dens[i] * a.inv_rho0);
+        const float cs = a.g7 ? fmaxf(a.c0f * ((r * r) * r), 1.0e-8f)
+                              : sqrtf(fmaxf(a.c0f * a.c0f * powf(r, a.gm1), 1.0e-16f));
         double ad[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) ad[d] = ks * accp[d] + kv * accv[d];
-        a.divv[i] = -(c0 / h) * divv * (1.0 / ((double)rho * a.inv_rho0));
+        for (int d = 0; d < D; ++d) ad[d] = a.ks * accp[d] + a.kv * accv[d];
+        // div v = -(c0 / h) sum / (rho / rho0), and dt = C_CFL h / (c_i + v_sig c0), in FP32
+        a.divv[i] = (double)(a.kdivv * divv * __builtin_amdgcn_rcpf(r));
         const float vs = fmaxf((float)vsig2.x, (float)vsig2.y);
-        const double dti = a.c_cfl * h / ((double)cs + (double)vs * c0);
+        const double dti = (double)(a.cfl_h * __builtin_amdgcn_rcpf(fmaf(vs, a.c0f, cs)));
         a.dti[i] = dti;
-        if (a.fused_final) cand = final_particle<D>(a.fin, a.s, i, n, ad, dti);
+        if (a.fused_final) cand = final_particle_h<D>(a.fin, a.s, i, n, ad, dti, a.hf, a.t2);
 #pragma unroll
         for (int d = 0; d < D; ++d) a.acc[d * n + i] = ad[d];
         --total;  // the particle itself passed the predicate (r = 0) and added zero

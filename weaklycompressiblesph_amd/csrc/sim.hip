@@ -550,9 +550,6 @@ static NFMixedArgs nforce_mixed_args(const wcsph_sim* s) {
     a.malpha = (float)(-2.0 * s->p.alpha);
     a.beta = (float)(2.0 * s->p.beta);
     a.k_Ls = splat(s->grid.L / (double)s->hmax);
-    a.c0 = s->p.c0;
-    a.nu = s->p.nu;
-    a.c_cfl = s->p.c_cfl;
     a.inv_rho0 = 1.0 / s->p.rho0;
     a.acc = s->acc;
     a.divv = s->divv;
@@ -561,6 +558,17 @@ static NFMixedArgs nforce_mixed_args(const wcsph_sim* s) {
     a.s = s->sc;
     a.fused_final = s->fuse_final ? 1 : 0;
     a.fin = final_args(s, 1);
+    // the epilogue's constants, formed once here rather than per particle
+    const double c0 = s->p.c0, h = s->hmax;
+    a.ks = c0 * c0 / h;
+    a.kv = 4.0 * s->p.nu * c0 / (h * h);
+    const double hd = s->h_const > 0.0 ? s->h_const : h;  // the particles' FP64 smoothing length
+    a.t2 = 0.125 * hd * hd / s->p.nu;
+    a.kdivv = (float)(-(c0 / h));
+    a.c0f = (float)c0;
+    a.cfl_h = (float)(s->p.c_cfl * h);
+    a.hf = (float)hd;
+    a.g7 = s->p.gamma == 7.0 ? 1 : 0;
     return a;
 }
 

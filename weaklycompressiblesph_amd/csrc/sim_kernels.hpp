@@ -891,6 +891,33 @@ __device__ __forceinline__ double final_particle(const FinalArgs& f, const Scala
     return cand;
 }
 
+// final_particle for one smoothing length hf with the viscous bound t2 =
+// 0.125 h^2 / nu formed on the host; the acceleration bound 0.25 sqrt(h/|a|)
+// is taken in FP32 (the kick and the density floor stay FP64).
+template <int D>
+__device__ __forceinline__ double final_particle_h(const FinalArgs& f, const Scalars* s, int64_t i, int64_t n,
+                                                   double* a, double dti, float hf, double t2) {
+    a[D - 1] -= f.g;
+    const bool fluid = f.m.type[i] == 0;
+    const double hdt = 0.5 * s->dt;
+    float a2 = 0.f;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const float af = (float)a[d];
+        a2 = fmaf(af, af, a2);
+        if (f.kick && fluid) f.m.vel[d * n + i] = f.m.vh[d * n + i] + hdt * a[d];
+    }
+    if (f.m.dens[i] < f.rho_floor) f.m.dens[i] = f.rho_floor;
+    double cand = 1.0e30;
+    if (fluid) {
+        const double t1 = (double)(0.25f * __builtin_amdgcn_sqrtf(hf * __builtin_amdgcn_rsqf(a2)));
+        cand = dti;
+        cand = t1 < cand ? t1 : cand;
+        cand = t2 < cand ? t2 : cand;
+    }
+    return cand;
+}
+
 // Minimum over the block of positive doubles (they order like their bit
 // patterns) into *slot. Every thread of the block calls it.
 __device__ __forceinline__ void block_min_store(double cand, double* slot) {
