@@ -420,9 +420,18 @@ struct EosConsts {
 // and the sound speed of density rho, in FP64.
 __device__ __forceinline__ void tait(const EosConsts& e, double rho, double& pr, double& cs) {
     const double r = rho / e.rho0;
-    double P = e.B * (pow(r, e.gamma) - 1.0);
+    double rg, rg1;  // (rho/rho0)^gamma and ^(gamma - 1)
+    if (e.gamma == 7.0) {  // the water law: by multiplies
+        const double r3 = (r * r) * r;
+        rg1 = r3 * r3;
+        rg = rg1 * r;
+    } else {
+        rg = pow(r, e.gamma);
+        rg1 = pow(r, e.gamma - 1.0);
+    }
+    double P = e.B * (rg - 1.0);
     P = P > 0.0 ? P : 0.0;
-    double c2 = e.B * e.gamma / e.rho0 * pow(r, e.gamma - 1.0);
+    double c2 = e.B * e.gamma / e.rho0 * rg1;
     c2 = c2 > 1.0e-16 ? c2 : 1.0e-16;
     pr = P / (rho * rho);
     cs = sqrt(c2);
