@@ -164,8 +164,7 @@ void run_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const PassG
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)npart), blk, 0, st, count, ncell + 1, tile_total, group_total, gsize,
                        start);
     hipLaunchKernelGGL(k_scatter, grid_n, blk, 0, st, key, slot, start, perm, count, n);
-    const double B = p->c0 * p->c0 * p->rho0 / p->gamma;
-    const EosConsts e{p->rho0, B, p->gamma};
+    const EosConsts e = eos_consts(p->rho0, p->c0, p->gamma);
     if (D == 3)
         hipLaunchKernelGGL(k_pass_records<3>, grid_n, blk, 0, st, perm, key, start, pos, vel, dens, mass, h, n, pg.g,
                            e, pk, id, sc);

@@ -333,7 +333,6 @@ static ScatterArgs scatter_args(const wcsph_sim* s) {
 
 template <int D, class Extra>
 static ReorderArgs<D, Extra> reorder_args(const wcsph_sim* s, int kick, Extra extra) {
-    const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
     const bool mixed = s->cfg.precision == WCSPH_PREC_MIXED;
     ReorderArgs<D, Extra> a{};
     a.perm = s->perm;
@@ -349,7 +348,7 @@ static ReorderArgs<D, Extra> reorder_args(const wcsph_sim* s, int kick, Extra ex
     a.g = s->grid;
     a.pr64 = s->pr64;
     a.c64 = s->c64;
-    a.e = EosConsts{s->p.rho0, B, s->p.gamma};
+    a.e = eos_consts(s->p.rho0, s->p.c0, s->p.gamma);
     a.v_scale = mixed ? (float)(1.0 / s->p.c0) : 1.0f;
     a.h_const = s->h_const;
     a.m_const = s->m_const;
@@ -445,7 +444,7 @@ __global__ void k_fp32_records(Master m, Packed pk, Grid g, EosConsts e, int64_t
 }
 
 static void fp32_records(wcsph_sim* s) {
-    const EosConsts e{s->p.rho0, s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma, s->p.gamma};
+    const EosConsts e = eos_consts(s->p.rho0, s->p.c0, s->p.gamma);
     if (s->D == 3)
         hipLaunchKernelGGL(k_fp32_records<3>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, s->m[s->cur], s->pk, s->grid,
                            e, s->n);
@@ -664,8 +663,7 @@ static void shepard_pass_d(wcsph_sim* s) {
             hipLaunchKernelGGL((k_shepard32<D, 2>), grid, block, 0, s->st, s->pk, s->start, s->grid, n, s->p.rho0,
                                m.dens, s->rho_tmp, s->sc);
     }
-    const double B = s->p.c0 * s->p.c0 * s->p.rho0 / s->p.gamma;
-    EosConsts e{s->p.rho0, B, s->p.gamma};
+    const EosConsts e = eos_consts(s->p.rho0, s->p.c0, s->p.gamma);
     if (s->cfg.precision == WCSPH_PREC_MIXED)
         hipLaunchKernelGGL((k_apply_density<D, MixedExtra<D>>), grid, block, 0, s->st, m, s->pk, s->rho_tmp,
                            s->pr64, s->c64, e, MixedExtra<D>{s->mx, s->mc}, n);

@@ -414,24 +414,34 @@ static __global__ void k_scatter(const uint32_t* __restrict__ key, const uint32_
 
 struct EosConsts {
     double rho0, B, gamma;
+    double inv_rho0, c0;  // 1 / rho0 and c0 = sqrt(B gamma / rho0)
 };
 
+// The Tait constants of rest density rho0, sound speed c0 and exponent
+// gamma: B = c0^2 rho0 / gamma (src/init/Dam2.h:38-43).
+inline EosConsts eos_consts(double rho0, double c0, double gamma) {
+    return EosConsts{rho0, c0 * c0 * rho0 / gamma, gamma, 1.0 / rho0, c0};
+}
+
 // Murnaghan/Tait (src/EoS.h:78-86, CalcPressure src/force.h:3-9): P/rho^2
-// and the sound speed of density rho, in FP64.
+// and the sound speed of density rho, in FP64. For the water law (gamma =
+// 7) the powers are multiplies and c = c0 (rho/rho0)^3, the square root of
+// c^2 = B gamma / rho0 (rho/rho0)^6 taken in closed form.
 __device__ __forceinline__ void tait(const EosConsts& e, double rho, double& pr, double& cs) {
-    const double r = rho / e.rho0;
-    double rg, rg1;  // (rho/rho0)^gamma and ^(gamma - 1)
-    if (e.gamma == 7.0) {  // the water law: by multiplies
+    if (e.gamma == 7.0) {
+        const double r = rho * e.inv_rho0;
         const double r3 = (r * r) * r;
-        rg1 = r3 * r3;
-        rg = rg1 * r;
-    } else {
-        rg = pow(r, e.gamma);
-        rg1 = pow(r, e.gamma - 1.0);
+        double P = e.B * ((r3 * r3) * r - 1.0);
+        P = P > 0.0 ? P : 0.0;
+        pr = P / (rho * rho);
+        const double c = e.c0 * r3;
+        cs = c > 1.0e-8 ? c : 1.0e-8;  // c^2 >= 1e-16
+        return;
     }
-    double P = e.B * (rg - 1.0);
+    const double r = rho / e.rho0;
+    double P = e.B * (pow(r, e.gamma) - 1.0);
     P = P > 0.0 ? P : 0.0;
-    double c2 = e.B * e.gamma / e.rho0 * rg1;
+    double c2 = e.B * e.gamma / e.rho0 * pow(r, e.gamma - 1.0);
     c2 = c2 > 1.0e-16 ? c2 : 1.0e-16;
     pr = P / (rho * rho);
     cs = sqrt(c2);
