@@ -553,6 +553,20 @@ def main():
             "step_pairs_per_s": runs["mixed"]["value"] / runs["fp32"]["value"],
             "force_kernel_pairs_per_s": runs["mixed"]["force_kernel_pairs_per_s"] /
                                         runs["fp32"]["force_kernel_pairs_per_s"]}
+    if not args.only:
+        # the same steps with frozen walls (wall_mode 1): waves of walls alone skip their wall-wall pairs,
+        # which add nothing the step uses; every fluid result and dt is the same. Pairs here are the pairs
+        # evaluated, so the figure is the step time, not the headline's pair rate.
+        fw = {}
+        for prec in ("mixed", "fp32"):
+            pw = W.dam_break_params(dim, shepard_interval=args.shepard, wall_mode=1)
+            sim, r = run_path(W, pw, st, prec, args)
+            sim.close()
+            fw[prec] = {"ms_per_step": r["ms_per_step"], "steps_per_s": r["steps_per_s"],
+                        "force_ms_per_launch": r["force_ms_per_launch"],
+                        "pairs_evaluated_per_step": r["pairs_per_step"],
+                        "speedup_per_step_vs_wall_mode_0": runs[prec]["ms_per_step"] / r["ms_per_step"]}
+        line["frozen_walls"] = fw
     if dim == 3 and abs(dx - 0.0126) < 1e-12:
         prof = profiled_traffic(kname)
         if prof:

@@ -75,7 +75,17 @@ typedef struct {
                            2 max(h_i, h_j). The two agree when h is uniform. The FDPS drop-in
                            takes either; the device-resident paths and wcsph_force_pass take
                            1 only with one smoothing length for all particles. */
-    int32_t pad_;
+    int32_t wall_mode;  /* device-resident step only (the FDPS drop-in and wcsph_force_pass take 0):
+                           0: every pair is evaluated, as the reference's functor does;
+                           1: frozen walls. A wall particle never moves and is left out of the
+                           global time step (src/class.h:146-163, src/integral.h:6), so its
+                           only result the step uses is div v, and a wall-wall pair adds
+                           exactly zero to it (dv = 0). A wave whose 64 particles are all
+                           walls then takes fluid neighbours only, and wall accelerations
+                           are written as zero. Every fluid result and every dt is the same;
+                           a wall's div v is the same sum (the FP32 path: the same bits; the
+                           mixed path: its terms may be added in another order). Needs
+                           every wall at rest (checked on upload and restore). */
 } wcsph_params;
 
 /* Fill p with the reference CPU functor's constants (src/force.h:32,45-47,

@@ -59,14 +59,15 @@ def test_ctypes_structs_match_header(W, tmp_path):
 #include <stddef.h>
 #include "wcsph_abi.h"
 int main(void) { printf("%zu %zu %zu %zu %zu\n", sizeof(wcsph_params), sizeof(wcsph_sim_config),
-    sizeof(wcsph_sim_stats), sizeof(wcsph_dam_break_desc), offsetof(wcsph_params, device)); return 0; }
+    sizeof(wcsph_sim_stats), sizeof(wcsph_dam_break_desc), offsetof(wcsph_params, device));
+    printf("%zu\n", offsetof(wcsph_params, wall_mode)); return 0; }
 ''')
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     a = W._abi
     assert got == [C.sizeof(a.Params), C.sizeof(a.SimConfig), C.sizeof(a.SimStats), C.sizeof(a.DamBreakDesc),
-                   a.Params.device.offset]
+                   a.Params.device.offset, a.Params.wall_mode.offset]
     for dim in (2, 3):
         assert W.epi_dtype(dim).itemsize == (96 if dim == 3 else 80)
         assert W.epj_dtype(dim).itemsize == (112 if dim == 3 else 96)
@@ -114,6 +115,14 @@ def test_error_paths_without_gpu(W):
     h = C.c_void_p()
     assert lib.wcsph_sim_create(C.byref(p), None, 10, C.byref(h)) == W._abi.WCSPH_EINVAL
     assert not h.value
+    # wall_mode: 0 or 1 on the device-resident step, 0 only for the FDPS drop-in
+    p = W.reference_params(3)
+    p.wall_mode = 2
+    assert lib.wcsph_sim_create(C.byref(p), None, 10, C.byref(h)) == W._abi.WCSPH_EINVAL
+    assert "wall_mode" in W.last_error() and not h.value
+    p.wall_mode = 1
+    assert lib.wcsph_init(C.byref(p)) == W._abi.WCSPH_EINVAL
+    assert "wall_mode" in W.last_error()
     with pytest.raises(W.WcsphError):
         W.check(lib.wcsph_sim_step(None, 1))
     bad = W.dam_break_desc(3, -1.0)

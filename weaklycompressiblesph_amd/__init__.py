@@ -38,15 +38,17 @@ def reference_params(dim, device=0):
 
 def dam_break_params(dim, *, col_h=2.0, alpha=0.1, beta=0.0, c_cfl=0.25, g=9.81, rho0=1000.0, gamma=7.0,
                      nu=1.0e-6, c0=None, rho_floor=0.0, vsig_mode=0, eps=0.01, shepard_interval=0, device=0,
-                     grad_mode=0):
+                     grad_mode=0, wall_mode=0):
     """Physics of the dam-break configurations: c0 = 10 sqrt(2 g H), alpha = 0.1, CFL 0.25, g = 9.81;
-    the Shepard filter is off unless shepard_interval > 0 (the reference uses 30)."""
+    the Shepard filter is off unless shepard_interval > 0 (the reference uses 30); wall_mode 1 is
+    the frozen-wall step (wcsph_params.wall_mode)."""
     p = reference_params(dim, device)
     p.alpha, p.beta, p.c_cfl, p.g, p.rho0, p.gamma, p.nu = alpha, beta, c_cfl, g, rho0, gamma, nu
     p.c0 = 10.0 * np.sqrt(2.0 * g * col_h) if c0 is None else c0
     p.rho_floor, p.vsig_mode, p.eps = rho_floor, vsig_mode, eps
     p.shepard_interval = shepard_interval
     p.grad_mode = grad_mode
+    p.wall_mode = wall_mode
     return p
 
 

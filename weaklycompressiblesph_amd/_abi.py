@@ -29,7 +29,7 @@ class Params(C.Structure):
                 ("eps", C.c_double), ("c_cfl", C.c_double), ("rho0", C.c_double), ("c0", C.c_double),
                 ("gamma", C.c_double), ("g", C.c_double), ("nu", C.c_double), ("rho_floor", C.c_double),
                 ("device", C.c_int32), ("shepard_interval", C.c_int32), ("grad_mode", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("wall_mode", C.c_int32)]
 
 
 class SimConfig(C.Structure):

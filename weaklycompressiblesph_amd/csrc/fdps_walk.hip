@@ -746,6 +746,8 @@ int backend_init(const wcsph_params* p) {
         WCSPH_REQUIRE(p != nullptr, WCSPH_EINVAL, "null params");
         WCSPH_REQUIRE(p->dim == 2 || p->dim == 3, WCSPH_EINVAL, "dim must be 2 or 3");
         WCSPH_REQUIRE(p->grad_mode == 0 || p->grad_mode == 1, WCSPH_EINVAL, "grad_mode must be 0 or 1");
+        WCSPH_REQUIRE(p->wall_mode == 0, WCSPH_EINVAL,
+                      "the FDPS drop-in evaluates every pair (wall_mode 0); wall_mode 1 is for the device-resident step");
         Backend& B = backend();
         std::lock_guard<std::mutex> lk(B.mu);
         int ndev = 0;

@@ -10,7 +10,9 @@
 //   PosRec16   {x, y, z, x-cell}   position relative to the cell centre / h
 //                                  (|x| <= L/2h, where FP16 has its finest
 //                                  steps) and the cell's x index
-//   VelRec16   {vx, vy, vz, m}     v / c0 and the scaled mass
+//   VelRec16   {vx, vy, vz, m}     v / c0 and the scaled mass, negated for a
+//                                  wall particle (the pair body masks the
+//                                  sign off with the hit bits)
 //   StateRec16 {rho[2], pr[2]}     FP32 rho / rho0 and (P / rho^2)(rho0 / c0^2)
 //   ch[m]                          half2 c / (2 c0), an array of its own
 // The three 16-byte records share one offset (m * 16) per candidate.
@@ -88,7 +90,7 @@ __device__ __forceinline__ void write_mixed_state(const Mixed& mx, const MixedCo
 template <int D>
 __device__ __forceinline__ void write_mixed_pair(const Mixed& mx, const MixedConsts& mc, int64_t p, bool live,
                                                  const float* xl, int cx, const float* vf, double rho, double pr,
-                                                 double c, double m) {
+                                                 double c, double m, bool wall) {
     uint32_t mine[4] = {0u, 0u, 0u, 0u};
     uint32_t fs[3] = {0u, 0u, 0u};  // rho, pr (FP32 bits) and c/2 (FP16 bits)
     if (live) {
@@ -96,7 +98,7 @@ __device__ __forceinline__ void write_mixed_pair(const Mixed& mx, const MixedCon
             h2{(_Float16)(xl[0] * mc.inv_h - mc.half_Ls), (_Float16)(xl[1] * mc.inv_h - mc.half_Ls)},
             h2{(_Float16)(D == 3 ? xl[2] * mc.inv_h - mc.half_Ls : 0.f), (_Float16)(float)cx},
             h2{(_Float16)vf[0], (_Float16)vf[1]},
-            h2{(_Float16)(D == 3 ? vf[2] : 0.f), (_Float16)(float)(m * mc.m_scale)}};
+            h2{(_Float16)(D == 3 ? vf[2] : 0.f), (_Float16)(float)(wall ? -(m * mc.m_scale) : m * mc.m_scale)}};
 #pragma unroll
         for (int k = 0; k < 4; ++k) mine[k] = as_u32(v[k]);
         fs[0] = __builtin_bit_cast(uint32_t, (float)(rho * mc.inv_rho0));
@@ -133,8 +135,8 @@ struct MixedExtra {  // k_reorder / k_apply_density hook writing the pair record
     Mixed mx;
     MixedConsts mc;
     __device__ void store_pair(int64_t p, bool live, const float* xl, int cx, const float* vf, double rho,
-                               double pr, double c, double m) const {
-        write_mixed_pair<D>(mx, mc, p, live, xl, cx, vf, rho, pr, c, m);
+                               double pr, double c, double m, bool wall) const {
+        write_mixed_pair<D>(mx, mc, p, live, xl, cx, vf, rho, pr, c, m, wall);
     }
     __device__ void state(int64_t p, double rho, double pr, double c) const {
         write_mixed_state(mx, mc, p, rho, pr, c);
@@ -183,6 +185,7 @@ struct NFMixedArgs {
     Scalars* s;
     int fused_final;  // run the final stage (FinalArgs) in the epilogue
     FinalArgs fin;
+    int wall_mode;  // frozen walls: a wave of walls alone takes fluid neighbours only
 };
 
 // Tuning constants (build-time overridable): stack entries per lane and
@@ -233,7 +236,8 @@ __device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& 
     constexpr uint32_t kRecMask = (1u << kMixRecBits) - 1u;
     static_assert(NR <= 32, "row index of a stack entry has 5 bits");
     // per (row, hit bits) of a stack entry: the row's (dy L/h, dz L/h) as
-    // half2 splats, the half2 mask keeping the hit halves, the hit count
+    // half2 splats, the half2 mask keeping the hit halves (sign bits off),
+    // the hit count
     __shared__ uint4 etab[128];
     // slot-major stacks: slot t of lane l at [t][l], so lanes reading
     // different slots still hit different banks
@@ -242,7 +246,8 @@ __device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& 
         const int row = (int)threadIdx.x & 31, bits = (int)threadIdx.x >> 5;
         const h2 dy = hsplat((float)(row % S - R)) * a.k_Ls;
         const h2 dz = hsplat(D == 3 ? (float)(row / S - R) : 0.f) * a.k_Ls;
-        const uint32_t sel = ((bits & 1) ? 0x0000ffffu : 0u) | ((bits & 2) ? 0xffff0000u : 0u);
+        // hit halves, sign bits off: the mass's sign is the wall flag
+        const uint32_t sel = ((bits & 1) ? 0x00007fffu : 0u) | ((bits & 2) ? 0x7fff0000u : 0u);
         etab[threadIdx.x] = make_uint4(as_u32(dy), as_u32(dz), sel, (uint32_t)__builtin_popcount(bits));
     }
     __syncthreads();
@@ -340,7 +345,7 @@ __device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& 
         // (r = 0) or one far outside the support (r^2 may overflow FP16);
         // the kernel weight is zero beyond r^2 = 4 either way
         r2 = hmin(hmax(r2, tiny2), sixteen2);
-        if (VS == 1) drdv = as_h2(as_u32(drdv) & T.z);
+        if (VS == 1) drdv = as_h2(as_u32(drdv) & (T.z | ((T.z << 1) & 0x80008000u)));  // whole hit halves
         const h2 mj = as_h2(as_u32(V.m) & T.z);
         const h2 inv_den = hrcp(r2 + eps2);     // 1 / (r^2 + eps h_i h_j)
         const h2 mu = hmin(drdv, zero2) * inv_den;  // (h_i + h_j)/2 = 1
@@ -417,55 +422,80 @@ __device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& 
         b0 = ok ? own + srow + xa : 0u;
         b1 = ok ? own + srow + xb + 1 : 0u;
     };
-    uint32_t nb0, nb1;
-    row_bounds(0, nb0, nb1);
-    uint32_t np0 = a.start[nb0], npe = a.start[nb1];
-    for (int r = 0; r < NR; ++r) {
-        const uint32_t p0 = np0, pend = npe;
-        if (r + 1 < NR) {  // prefetch the next row's range behind this row's scan
-            row_bounds(r + 1, nb0, nb1);
-            np0 = a.start[nb0];
-            npe = a.start[nb1];
-        }
-        const uint4 T = etab[r];
-        const h2 ry = yi2 - as_h2(T.x), rz = zi2 - as_h2(T.y);
-        // record k of the window holds particles 2 (m0 + k) and 2 (m0 + k) + 1;
-        // its low half lies in [p0, pend) for k < npair except k = 0 when p0
-        // is odd, its high half for k < hi_lim (an empty window starting at
-        // an odd index scans one record with neither half valid)
-        const uint32_t m0 = p0 >> 1;
-        const uint32_t npair = ((pend + 1) >> 1) - m0;
-        const uint32_t hi_lim = (pend >> 1) - m0;
-        const bool even_start = (p0 & 1) == 0;
-        const uint32_t ebase = m0 | ((uint32_t)r << kMixRecBits);
-        const int32_t voff = (int32_t)(m0 * (uint32_t)sizeof(PosRec16));
-        const uint32_t npmax = wave_max(npair);
-        for (uint32_t t = 0; t < npmax; t += U) {
-            PosRec16 rec[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-                rec[u] = __builtin_bit_cast(
-                    PosRec16, __builtin_amdgcn_raw_buffer_load_b128(
-                                  pos_rsrc, voff + (int32_t)((t + u) * sizeof(PosRec16)), 0, 0));
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t k = t + u;
-                const h2 dx = hfma(cxi2 - rec[u].cx, Ls2, xi2) - rec[u].x;
-                const h2 dy = ry - rec[u].y;
-                const h2 dz = rz - rec[u].z;
-                h2 r2 = dx * dx;
-                r2 = hfma(dy, dy, r2);
-                if (D == 3) r2 = hfma(dz, dz, r2);
-                const bool lo = k < npair && (k != 0 || even_start) && r2.x < four;
-                const bool hi = k < hi_lim && r2.y < four;
-                // the top slot is free (at most Q - U entries before this
-                // trip), so write it always and keep it only for a record
-                // with a hit
-                *qp = (ebase + k) | (lo ? 1u << 30 : 0u) | (hi ? 1u << 31 : 0u);
-                qp += (lo || hi) ? 64 : 0;
+    // The rows' records; with walls_only (a wave of frozen walls, below)
+    // only fluid halves are kept: a wall-wall pair adds exactly zero to
+    // div v (dv = 0) and to v_sig (mu = 0), and a wall's acceleration is
+    // not used. The stack then holds fewer entries, so the kept terms may
+    // be added in another order.
+    auto scan_rows = [&](auto walls_only) {
+        constexpr bool WO = decltype(walls_only)::value;
+        constexpr int UW = WO ? 1 : U;  // records per trip (one in a wall wave: fewer registers)
+        uint32_t nb0, nb1;
+        row_bounds(0, nb0, nb1);
+        uint32_t np0 = a.start[nb0], npe = a.start[nb1];
+        for (int r = 0; r < NR; ++r) {
+            const uint32_t p0 = np0, pend = npe;
+            if (r + 1 < NR) {  // prefetch the next row's range behind this row's scan
+                row_bounds(r + 1, nb0, nb1);
+                np0 = a.start[nb0];
+                npe = a.start[nb1];
             }
-            if (__any(qp > qlim)) drain();
+            const uint4 T = etab[r];
+            const h2 ry = yi2 - as_h2(T.x), rz = zi2 - as_h2(T.y);
+            // record k of the window holds particles 2 (m0 + k) and 2 (m0 + k) + 1;
+            // its low half lies in [p0, pend) for k < npair except k = 0 when p0
+            // is odd, its high half for k < hi_lim (an empty window starting at
+            // an odd index scans one record with neither half valid)
+            const uint32_t m0 = p0 >> 1;
+            const uint32_t npair = ((pend + 1) >> 1) - m0;
+            const uint32_t hi_lim = (pend >> 1) - m0;
+            const bool even_start = (p0 & 1) == 0;
+            const uint32_t ebase = m0 | ((uint32_t)r << kMixRecBits);
+            const int32_t voff = (int32_t)(m0 * (uint32_t)sizeof(PosRec16));
+            const uint32_t npmax = wave_max(npair);
+            for (uint32_t t = 0; t < npmax; t += UW) {
+                PosRec16 rec[UW];
+                uint32_t tw[UW];  // the mass halves, whose signs flag walls (walls_only)
+#pragma unroll
+                for (int u = 0; u < UW; ++u) {
+                    rec[u] = __builtin_bit_cast(
+                        PosRec16, __builtin_amdgcn_raw_buffer_load_b128(
+                                      pos_rsrc, voff + (int32_t)((t + u) * sizeof(PosRec16)), 0, 0));
+                    tw[u] = WO ? __builtin_amdgcn_raw_buffer_load_b32(
+                                     vel_rsrc, voff + (int32_t)((t + u) * sizeof(VelRec16) + 12), 0, 0)
+                               : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < UW; ++u) {
+                    const uint32_t k = t + u;
+                    const h2 dx = hfma(cxi2 - rec[u].cx, Ls2, xi2) - rec[u].x;
+                    const h2 dy = ry - rec[u].y;
+                    const h2 dz = rz - rec[u].z;
+                    h2 r2 = dx * dx;
+                    r2 = hfma(dy, dy, r2);
+                    if (D == 3) r2 = hfma(dz, dz, r2);
+                    const bool lo = k < npair && (k != 0 || even_start) && r2.x < four &&
+                                    (!WO || (tw[u] & 0x8000u) == 0u);
+                    const bool hi = k < hi_lim && r2.y < four && (!WO || (tw[u] >> 31) == 0u);
+                    // the top slot is free (at most Q - U entries before this
+                    // trip), so write it always and keep it only for a record
+                    // with a hit
+                    *qp = (ebase + k) | (lo ? 1u << 30 : 0u) | (hi ? 1u << 31 : 0u);
+                    qp += (lo || hi) ? 64 : 0;
+                }
+                if (__any(qp > qlim)) drain();
+            }
         }
+    };
+    // frozen walls (wall_mode 1): a wave whose particles are all walls
+    const bool wall_i = ((as_u32(Vi.m) >> (odd ? 31 : 15)) & 1u) != 0u;
+    if (a.wall_mode && __all(!live || wall_i)) {
+        scan_rows(BoolTag<true>{});
+    } else {
+        // the particle itself passes the predicate (r = 0) and adds zero:
+        // it is not a pair
+        total = live ? ~0u : 0u;
+        scan_rows(BoolTag<false>{});
     }
     while (__any(qp != q)) drain();
     double cand = failed ? 0.0 : 1.0e30;  // a diverged state stops the clock
@@ -477,8 +507,9 @@ __device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& 
         const float cs = a.g7 ? fmaxf(a.c0f * ((r * r) * r), 1.0e-8f)
                               : sqrtf(fmaxf(a.c0f * a.c0f * powf(r, a.gm1), 1.0e-16f));
         double ad[D];
+        const bool frozen = a.wall_mode && a.fin.m.type[i] != 0;  // a frozen wall's acceleration reads as zero
 #pragma unroll
-        for (int d = 0; d < D; ++d) ad[d] = a.ks * accp[d] + a.kv * accv[d];
+        for (int d = 0; d < D; ++d) ad[d] = frozen ? 0.0 : a.ks * accp[d] + a.kv * accv[d];
         // div v = -(c0 / h) sum / (rho / rho0), and dt = C_CFL h / (c_i + v_sig c0), in FP32
         a.divv[i] = (double)(a.kdivv * divv * __builtin_amdgcn_rcpf(r));
         const float vs = fmaxf((float)vsig2.x, (float)vsig2.y);
@@ -487,7 +518,6 @@ __device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& 
         if (a.fused_final) cand = final_particle_h<D>(a.fin, a.s, i, n, ad, dti, a.hf, a.t2);
 #pragma unroll
         for (int d = 0; d < D; ++d) a.acc[d * n + i] = ad[d];
-        --total;  // the particle itself passed the predicate (r = 0) and added zero
     }
     block_count_atomic(total, a.s, true, a.fused_final ? &a.P.dtmin[vb.id] : nullptr, cand);
 }

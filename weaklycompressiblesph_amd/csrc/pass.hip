@@ -97,6 +97,8 @@ void check_pass_args(const wcsph_params* p, const wcsph_pass_desc* d, const doub
     WCSPH_REQUIRE(d->n > 0 && d->n < ((int64_t)1 << 27), WCSPH_EINVAL, "n must be in [1, 2^27)");
     WCSPH_REQUIRE(p->grad_mode == 0, WCSPH_EINVAL,
                   "the force pass takes the gradient at the mean smoothing length only (grad_mode 0)");
+    WCSPH_REQUIRE(p->wall_mode == 0, WCSPH_EINVAL,
+                  "the force pass evaluates every pair (wall_mode 0); wall_mode 1 is for the device-resident step");
     WCSPH_REQUIRE(pos && vel && dens && mass && h && acc && div_v && dt_i, WCSPH_EINVAL, "null array");
 }
 

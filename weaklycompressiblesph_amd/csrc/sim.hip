@@ -106,6 +106,7 @@ void check_params(const wcsph_params* p) {
                   "rho0, c0, gamma, c_cfl must be positive and eps non-negative");
     WCSPH_REQUIRE(p->shepard_interval >= 0, WCSPH_EINVAL, "shepard_interval must be non-negative");
     WCSPH_REQUIRE(p->grad_mode == 0 || p->grad_mode == 1, WCSPH_EINVAL, "grad_mode must be 0 or 1");
+    WCSPH_REQUIRE(p->wall_mode == 0 || p->wall_mode == 1, WCSPH_EINVAL, "wall_mode must be 0 or 1");
 }
 
 constexpr int kDiag = 10;  // diagnostic sums
@@ -401,6 +402,7 @@ static FinalArgs final_args(const wcsph_sim* s, int kick) {
     f.nu = s->p.nu;
     f.rho_floor = s->p.rho_floor;
     f.kick = kick;
+    f.wall_mode = s->p.wall_mode;
     return f;
 }
 
@@ -495,6 +497,7 @@ static void launch_force64(wcsph_sim* s) {
     a.nu = s->p.nu;
     a.c_cfl = s->p.c_cfl;
     a.k = pair_consts<double>(s);
+    a.wtype = s->p.wall_mode ? m.type : nullptr;
     hipLaunchKernelGGL(k_force64<D>, dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
 }
 
@@ -517,6 +520,7 @@ static void launch_nforce(wcsph_sim* s) {
     a.fused_final = s->fuse_final ? 1 : 0;
     a.fin = final_args(s, 1);
     a.out_index = nullptr;
+    a.wtype = s->p.wall_mode ? s->m[s->cur].type : nullptr;
     hipLaunchKernelGGL((k_nforce<D, R, Q, UH, VS>), dim3(s->nb(s->n)), dim3(kBlock), 0, s->st, a);
 }
 
@@ -568,6 +572,7 @@ static NFMixedArgs nforce_mixed_args(const wcsph_sim* s) {
     a.cfl_h = (float)(s->p.c_cfl * h);
     a.hf = (float)hd;
     a.g7 = s->p.gamma == 7.0 ? 1 : 0;
+    a.wall_mode = s->p.wall_mode;
     return a;
 }
 
@@ -900,6 +905,9 @@ int wcsph_sim_upload(wcsph_sim* s, const double* pos, const double* vel, const d
                 WCSPH_REQUIRE(std::isfinite(pos[d * n + i]) && std::isfinite(vel[d * n + i]), WCSPH_EINVAL,
                               "positions and velocities must be finite");
             WCSPH_REQUIRE(type[i] == 0 || type[i] == 1, WCSPH_EINVAL, "type must be 0 (fluid) or 1 (wall)");
+            if (s->p.wall_mode && type[i] == 1)
+                for (int d = 0; d < D; ++d)
+                    WCSPH_REQUIRE(vel[d * n + i] == 0.0, WCSPH_EINVAL, "wall_mode 1 needs every wall particle at rest");
         }
         build_grid(s, pos, h);
         setup_precision(s);
@@ -1440,6 +1448,9 @@ int wcsph_sim_restore(const char* path, int32_t device, wcsph_sim** out) {
             WCSPH_REQUIRE(hs.id[q] < (uint64_t)n && !seen[hs.id[q]], WCSPH_EIO, "restart file: ids are not a permutation");
             seen[hs.id[q]] = 1;
             WCSPH_REQUIRE(hs.type[q] <= 1, WCSPH_EIO, "restart file: bad particle type");
+            if (hd.params.wall_mode && hs.type[q] == 1)
+                for (int d = 0; d < hd.params.dim; ++d)
+                    WCSPH_REQUIRE(hs.vel[d * n + q] == 0.0, WCSPH_EIO, "restart file: wall_mode 1 with a moving wall");
         }
         wcsph_params p = hd.params;
         p.device = device;

@@ -529,6 +529,7 @@ __device__ __forceinline__ void reorder_body(int64_t blk, const uint32_t* __rest
     float xl[3] = {0.f, 0.f, 0.f}, vf[3] = {0.f, 0.f, 0.f};
     int c[3] = {0, 0, 0};
     double rho = 0.0, m = 0.0, pr = 0.0, cs = 0.0;
+    bool wall = false;
     if (live) {
         const uint32_t k = key[perm[p]];
         const uint2 cr = make_uint2(start[k], start[k + 1]);  // the cell's slot range
@@ -566,6 +567,7 @@ __device__ __forceinline__ void reorder_body(int64_t blk, const uint32_t* __rest
         if (!(m_const > 0.0)) dst.mass[p] = m;
         if (!(h_const > 0.0)) dst.h[p] = h;
         dst.type[p] = src.type[o];
+        wall = src.type[o] != 0;
         dst.id[p] = src.id[o];
         tait(e, rho, pr, cs);
         if (Extra::kFp32Records) {
@@ -579,7 +581,7 @@ __device__ __forceinline__ void reorder_body(int64_t blk, const uint32_t* __rest
             c64[p] = cs;
         }
     }
-    extra.store_pair(p, live, xl, c[0], vf, rho, pr, cs, m);  // every lane takes part
+    extra.store_pair(p, live, xl, c[0], vf, rho, pr, cs, m, wall);  // every lane takes part
 }
 
 template <int D, class Extra>
@@ -597,7 +599,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const uint32_t* __restrict__
 struct NoExtra {  // FP32 and FP64 paths: the FP32 records are all there is
     static constexpr bool kFp32Records = true;
     __device__ void store_pair(int64_t, bool, const float*, int, const float*, double, double, double,
-                               double) const {}
+                               double, bool) const {}
     __device__ void state(int64_t, double, double, double) const {}
 };
 
@@ -883,6 +885,7 @@ struct FinalArgs {
     Master m;
     double g, nu, rho_floor;
     int kick;
+    int wall_mode;  // 1: wall accelerations are written as zero (frozen walls)
 };
 
 template <int D>
@@ -906,6 +909,9 @@ __device__ __forceinline__ double final_particle(const FinalArgs& f, const Scala
         cand = dti;
         cand = t1 < cand ? t1 : cand;
         cand = t2 < cand ? t2 : cand;
+    } else if (f.wall_mode) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) a[d] = 0.0;
     }
     return cand;
 }
@@ -933,6 +939,9 @@ __device__ __forceinline__ double final_particle_h(const FinalArgs& f, const Sca
         cand = dti;
         cand = t1 < cand ? t1 : cand;
         cand = t2 < cand ? t2 : cand;
+    } else if (f.wall_mode) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) a[d] = 0.0;
     }
     return cand;
 }
@@ -972,6 +981,8 @@ struct NFArgs {
     FinalArgs fin;
     // outputs of cell-order particle i go to index out_index[i] (null: i)
     const uint32_t* out_index;
+    // frozen walls (wall_mode 1): the particle types in cell order; null: off
+    const uint8_t* wtype;
 };
 
 
@@ -993,6 +1004,11 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
 }
 
 constexpr int kScanUnroll = 4;  // candidate loads in flight per lane
+
+template <bool B>
+struct BoolTag {  // a compile-time flag passed to a generic lambda
+    static constexpr bool value = B;
+};
 
 template <int D, int R, int Q, bool UH, int VS>
 __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
@@ -1098,59 +1114,73 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
             }
         }
     };
-    uint32_t nb0, nb1;
-    row_bounds(0, nb0, nb1);
-    uint32_t np0 = nb1 ? a.start[nb0] : 0u, npe = nb1 ? a.start[nb1] : 0u;
-    for (int r = 0; r < NR; ++r) {
-        const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
-        const uint32_t p0 = np0, len = npe - np0;
-        if (r + 1 < NR) {  // prefetch the next row's range behind this row's scan
-            row_bounds(r + 1, nb0, nb1);
-            np0 = nb1 ? a.start[nb0] : 0u;
-            npe = nb1 ? a.start[nb1] : 0u;
-        }
-        // the y/z part of the predicate is fixed for the row
-        const float yi = fmaf((float)(-dy), Lf, xi[1]);
-        const float zi = D == 3 ? fmaf((float)(-dz), Lf, xi[2]) : 0.f;
-        const float HH = UH ? __fmul_rn(__fadd_rn(I.h, I.h), __fadd_rn(I.h, I.h)) : 0.f;
-        for (uint32_t t = 0; __any(t < len); t += kScanUnroll) {
-            // issue all loads of the group before testing any of them
-            float4 pj[kScanUnroll];
-            float hj[kScanUnroll];
-#pragma unroll
-            for (int u = 0; u < kScanUnroll; ++u) {
-                // an empty window may start at n (cells past the last particle): read record 0
-                const uint32_t j = len ? p0 + min(t + u, len - 1) : 0u;
-                pj[u] = a.pk.px[j];
-                hj[u] = UH ? I.h : a.pk.pq[j].w;
+    // The rows' candidates; with walls_only (a wave of frozen walls, below)
+    // only fluid candidates are kept: a wall-wall pair adds exactly zero to
+    // div v (dv = 0) and to v_sig (mu = 0), and a wall's acceleration is
+    // not used. The kept candidates keep their order, so every sum is the
+    // same.
+    auto scan_rows = [&](auto walls_only) {
+        constexpr bool WO = decltype(walls_only)::value;
+        uint32_t nb0, nb1;
+        row_bounds(0, nb0, nb1);
+        uint32_t np0 = nb1 ? a.start[nb0] : 0u, npe = nb1 ? a.start[nb1] : 0u;
+        for (int r = 0; r < NR; ++r) {
+            const int dy = r % S - R, dz = D == 3 ? r / S - R : 0;
+            const uint32_t p0 = np0, len = npe - np0;
+            if (r + 1 < NR) {  // prefetch the next row's range behind this row's scan
+                row_bounds(r + 1, nb0, nb1);
+                np0 = nb1 ? a.start[nb0] : 0u;
+                npe = nb1 ? a.start[nb1] : 0u;
             }
+            // the y/z part of the predicate is fixed for the row
+            const float yi = fmaf((float)(-dy), Lf, xi[1]);
+            const float zi = D == 3 ? fmaf((float)(-dz), Lf, xi[2]) : 0.f;
+            const float HH = UH ? __fmul_rn(__fadd_rn(I.h, I.h), __fadd_rn(I.h, I.h)) : 0.f;
+            for (uint32_t t = 0; __any(t < len); t += kScanUnroll) {
+                // issue all loads of the group before testing any of them
+                float4 pj[kScanUnroll];
+                float hj[kScanUnroll];
+                uint32_t tj[kScanUnroll];
 #pragma unroll
-            for (int u = 0; u < kScanUnroll; ++u) {
-                const float dx = __fsub_rn(fmaf(cxi - pj[u].w, Lf, xi[0]), pj[u].x);
-                const float dy = __fsub_rn(yi, pj[u].y);
-                float r2 = __fmul_rn(dx, dx);
-                r2 = fmaf(dy, dy, r2);
-                if (D == 3) r2 = fmaf(__fsub_rn(zi, pj[u].z), __fsub_rn(zi, pj[u].z), r2);
-                float H2 = HH;
-                if (!UH) {
-                    const float H = __fadd_rn(I.h, hj[u]);
-                    H2 = __fmul_rn(H, H);
+                for (int u = 0; u < kScanUnroll; ++u) {
+                    // an empty window may start at n (cells past the last particle): read record 0
+                    const uint32_t j = len ? p0 + min(t + u, len - 1) : 0u;
+                    pj[u] = a.pk.px[j];
+                    hj[u] = UH ? I.h : a.pk.pq[j].w;
+                    tj[u] = WO ? (uint32_t)a.wtype[j] : 0u;
                 }
-                if (t + u < len && r2 > 0.f && r2 < H2) {
-                    q[qn] = (p0 + t + u) | ((uint32_t)r << kShift);
-                    ++qn;
+#pragma unroll
+                for (int u = 0; u < kScanUnroll; ++u) {
+                    const float dx = __fsub_rn(fmaf(cxi - pj[u].w, Lf, xi[0]), pj[u].x);
+                    const float dy = __fsub_rn(yi, pj[u].y);
+                    float r2 = __fmul_rn(dx, dx);
+                    r2 = fmaf(dy, dy, r2);
+                    if (D == 3) r2 = fmaf(__fsub_rn(zi, pj[u].z), __fsub_rn(zi, pj[u].z), r2);
+                    float H2 = HH;
+                    if (!UH) {
+                        const float H = __fadd_rn(I.h, hj[u]);
+                        H2 = __fmul_rn(H, H);
+                    }
+                    if (t + u < len && r2 > 0.f && r2 < H2 && (!WO || tj[u] == 0u)) {
+                        q[qn] = (p0 + t + u) | ((uint32_t)r << kShift);
+                        ++qn;
+                    }
                 }
+                if (__any(qn + kScanUnroll > (uint32_t)Q)) flush();
             }
-            if (__any(qn + kScanUnroll > (uint32_t)Q)) flush();
         }
-    }
+    };
+    // frozen walls (wall_mode 1): a wave whose particles are all walls
+    if (a.wtype != nullptr && __all(!live || a.wtype[i] != 0)) scan_rows(BoolTag<true>{});
+    else scan_rows(BoolTag<false>{});
     flush();
     double cand = failed ? 0.0 : 1.0e30;  // a diverged state stops the clock
     if (live) {
         const int64_t n = a.n;
         double ad[D];
+        const bool frozen = a.wtype != nullptr && a.wtype[i] != 0;  // a frozen wall's acceleration reads as zero
 #pragma unroll
-        for (int d = 0; d < D; ++d) ad[d] = (double)acc[d];
+        for (int d = 0; d < D; ++d) ad[d] = frozen ? 0.0 : (double)acc[d];
         const double dti = a.c_cfl * (double)I.h / ((double)I.c + (double)vsig);
         const int64_t o = a.out_index ? (int64_t)a.out_index[i] : i;
         a.divv[o] = (double)divv / (double)I.rho;
@@ -1365,6 +1395,7 @@ struct ForceArgs64 {
     double nu;
     double c_cfl;
     PairConsts<double> k;
+    const uint8_t* wtype;  // frozen walls (wall_mode 1): wall accelerations written as zero; null: off
 };
 
 template <int D>
@@ -1400,8 +1431,9 @@ __global__ __launch_bounds__(kBlock) void k_force64(ForceArgs64 a) {
         const double den = pair_mu(r2, drdv, I, J.h, a.k, mu, vsig);
         pair_force<D>(dr, dv, r2, drdv, den, mu, I, J, a.k, acc, divv);
     }
+    const bool frozen = a.wtype != nullptr && a.wtype[i] != 0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) a.acc[d * n + i] = acc[d];
+    for (int d = 0; d < D; ++d) a.acc[d * n + i] = frozen ? 0.0 : acc[d];
     a.divv[i] = divv / I.rho;                               // src/force.h:57
     a.dti[i] = a.c_cfl * I.h / (I.c + vsig);               // src/force.h:56
 }
@@ -1420,6 +1452,8 @@ __global__ __launch_bounds__(kBlock) void k_final(FinalArgs f, double* __restric
         for (int d = 0; d < D; ++d) a[d] = acc[d * n + i];
         cand = final_particle<D>(f, s, i, n, a, dti[i]);
         acc[(D - 1) * n + i] = a[D - 1];
+        if (f.wall_mode && f.m.type[i] != 0)  // a frozen wall's acceleration reads as zero
+            for (int d = 0; d < D - 1; ++d) acc[d * n + i] = 0.0;
     }
     block_min_store(cand, &P.dtmin[blockIdx.x]);
 }
