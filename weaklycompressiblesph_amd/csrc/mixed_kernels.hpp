@@ -1,7 +1,8 @@
 // Mixed-precision fused neighbour + force pass (uniform smoothing length).
 //
 // Units: lengths in h, velocities in c0, densities in rho0, P/rho^2 in
-// c0^2/rho0, masses in rho0 h^D (times the kernel constant C_D/2^(D+1)).
+// c0^2/rho0, masses in rho0 h^D (times 3 C_D/2^(D+1): the kernel constant
+// and the 3 of the spline's derivative).
 // In these units every quantity of the pair body is O(1e-4..50) and fits
 // FP16 (in SI units 1/H^4 alone would overflow it).
 //
@@ -14,7 +15,9 @@
 //                                  wall particle (the pair body masks the
 //                                  sign off with the hit bits)
 //   StateRec16 {rho[2], pr[2]}     FP32 rho / rho0 and (P / rho^2)(rho0 / c0^2)
-//   ch[m]                          half2 c / (2 c0), an array of its own
+//   ch[m]                          half2 -alpha c / c0, an array of its own:
+//                                  the artificial viscosity's -2 alpha times
+//                                  c / (2 c0), so ch_i + ch_j is -2 alpha c_bar / c0
 // The three 16-byte records share one offset (m * 16) per candidate.
 // One 16-byte load therefore brings two candidates' positions, and the
 // geometric part of the pair body -- differences, r^2, dr.dv, the spline
@@ -69,7 +72,7 @@ __device__ __forceinline__ h2 as_h2(uint32_t u) { return __builtin_bit_cast(h2, 
 __device__ __forceinline__ uint32_t as_u32(h2 v) { return __builtin_bit_cast(uint32_t, v); }
 
 struct MixedConsts {
-    float inv_h, inv_c0, inv_rho0, pr_scale, m_scale;  // SI -> scaled units; m_scale includes C_D / 2^(D+1)
+    float inv_h, ch_scale, inv_rho0, pr_scale, m_scale;  // SI -> scaled units (ch_scale = -alpha / c0)
     float half_Ls;  // half the cell edge in h: positions are stored relative to the cell centre
 };
 
@@ -81,7 +84,7 @@ __device__ __forceinline__ void write_mixed_state(const Mixed& mx, const MixedCo
     StateRec16& S = mx.st[p >> 1];
     S.rho[o] = (float)(rho * mc.inv_rho0);
     S.pr[o] = (float)(pr * mc.pr_scale);
-    reinterpret_cast<_Float16*>(&mx.ch[p >> 1])[o] = (_Float16)(float)(0.5 * c * mc.inv_c0);
+    reinterpret_cast<_Float16*>(&mx.ch[p >> 1])[o] = (_Float16)(float)(c * mc.ch_scale);
 }
 
 // The pair records written whole by k_reorder: lanes p and p ^ 1 (one wave)
@@ -103,7 +106,7 @@ __device__ __forceinline__ void write_mixed_pair(const Mixed& mx, const MixedCon
         for (int k = 0; k < 4; ++k) mine[k] = as_u32(v[k]);
         fs[0] = __builtin_bit_cast(uint32_t, (float)(rho * mc.inv_rho0));
         fs[1] = __builtin_bit_cast(uint32_t, (float)(pr * mc.pr_scale));
-        fs[2] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)(float)(0.5 * c * mc.inv_c0));
+        fs[2] = (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)(float)(c * mc.ch_scale));
     }
     uint32_t other[4];
 #pragma unroll
@@ -147,11 +150,21 @@ __device__ __forceinline__ h2 hsplat(float v) {
     const _Float16 h = (_Float16)v;
     return h2{h, h};
 }
+// FP16 transcendentals of both halves into one register: the low half by
+// the plain op, the high half by its SDWA form writing word 1 in place (no
+// repacking); each is followed by the one wait state a read of a
+// transcendental's result needs.
 __device__ __forceinline__ h2 hrcp(h2 v) {
-    return h2{(_Float16)__builtin_amdgcn_rcph(v.x), (_Float16)__builtin_amdgcn_rcph(v.y)};
+    uint32_t r;
+    asm("v_rcp_f16_e32 %0, %1\n\ts_nop 0\n\tv_rcp_f16_sdwa %0, %1 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE src0_sel:WORD_1\n\ts_nop 0"
+        : "=&v"(r) : "v"(as_u32(v)));
+    return as_h2(r);
 }
 __device__ __forceinline__ h2 hrsq(h2 v) {
-    return h2{(_Float16)__builtin_amdgcn_rsqh(v.x), (_Float16)__builtin_amdgcn_rsqh(v.y)};
+    uint32_t r;
+    asm("v_rsq_f16_e32 %0, %1\n\ts_nop 0\n\tv_rsq_f16_sdwa %0, %1 dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE src0_sel:WORD_1\n\ts_nop 0"
+        : "=&v"(r) : "v"(as_u32(v)));
+    return as_h2(r);
 }
 __device__ __forceinline__ h2 hmax(h2 a, h2 b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ h2 hmin(h2 a, h2 b) { return __builtin_elementwise_min(a, b); }
@@ -168,7 +181,7 @@ struct NFMixedArgs {
     float h;
     // wave-uniform constants (kept in scalar registers)
     h2 k_eps, k_Ls;
-    float malpha, beta;  // -2 alpha, 2 beta
+    float beta;  // 2 beta (the -2 alpha is in the ch records)
     double inv_rho0;
     float gm1;           // gamma - 1 of the Tait law (sound speed of i in the epilogue)
     // epilogue constants, formed once on the host: the force and viscous
@@ -290,7 +303,6 @@ __device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& 
     const float rhoi = odd ? Si.rho[1] : Si.rho[0], pri = odd ? Si.pr[1] : Si.pr[0];
     const h2 chi2 = pick(a.mx.ch[ii >> 1]);
     const float beta = a.beta;
-    const h2 malpha2 = hsplat(a.malpha);
     const h2 Ls2 = a.k_Ls, eps2 = a.k_eps;
     const h2 zero2 = hsplat(0.f), tiny2 = hsplat(1.0e-6f), sixteen2 = hsplat(16.f);
     const _Float16 four = (_Float16)4.f;
@@ -350,13 +362,16 @@ __device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& 
         const h2 inv_den = hrcp(r2 + eps2);     // 1 / (r^2 + eps h_i h_j)
         const h2 mu = hmin(drdv, zero2) * inv_den;  // (h_i + h_j)/2 = 1
         if (VS == 1) vsig2 = hmax(vsig2, -mu);
-        const h2 cbar = chi2 + chj;
-        const h2 amu = BT ? hfma(malpha2, cbar, hsplat(beta) * mu) * mu : malpha2 * cbar * mu;  // Pi (rho_i + rho_j)
+        const h2 cbar = chi2 + chj;  // -2 alpha c_bar / c0
+        const h2 amu = BT ? hfma(hsplat(beta), mu, cbar) * mu : cbar * mu;  // Pi (rho_i + rho_j)
         const h2 invr = hrsq(r2);
         const h2 rr = r2 * invr;               // r / h
+        // the spline's derivative over q, 12 bb^2 - 3 aa^2 with aa = (1 - q/2)+
+        // and bb = (1/2 - q/2)+, taken as b2^2 - aa^2 with b2 = 2 bb = (1 - q)+
+        // (its 3 is in the scaled mass)
         const h2 aa = hmax(hfma(rr, hsplat(-0.5f), hsplat(1.f)), zero2);
-        const h2 bb = hmax(hfma(rr, hsplat(-0.5f), hsplat(0.5f)), zero2);
-        const h2 g = hfma(bb * bb, hsplat(12.f), (aa * aa) * hsplat(-3.f));
+        const h2 b2 = hmax(hsplat(1.f) - rr, zero2);
+        const h2 g = hfma(b2, b2, -(aa * aa));
         const h2 mF = mj * g * invr;           // m_j F (scaled)
         const h2 mFv = mF * (r2 * inv_den);    // m_j F r^2 / den, the viscous factor before 1/(rho_i + rho_j)
         // density and pressure terms in FP32, one half at a time

@@ -308,11 +308,11 @@ static void setup_precision(wcsph_sim* s) {
                   "the mixed-precision path holds at most 2^26 particles");
     const double hh = (double)s->hmax;
     s->mc.inv_h = (float)(1.0 / hh);
-    s->mc.inv_c0 = (float)(1.0 / s->p.c0);
+    s->mc.ch_scale = (float)(-s->p.alpha / s->p.c0);
     s->mc.inv_rho0 = (float)(1.0 / s->p.rho0);
     s->mc.pr_scale = (float)(s->p.rho0 / (s->p.c0 * s->p.c0));
     const double cnorm = (D == 2 ? 80.0 / (7.0 * M_PI) : 16.0 / M_PI) / (D == 2 ? 8.0 : 16.0);
-    s->mc.m_scale = (float)(cnorm / (s->p.rho0 * std::pow(hh, D)));
+    s->mc.m_scale = (float)(3.0 * cnorm / (s->p.rho0 * std::pow(hh, D)));  // with the spline derivative's 3
     s->mc.half_Ls = (float)(0.5 * s->grid.L / hh);
 }
 
@@ -550,7 +550,6 @@ static NFMixedArgs nforce_mixed_args(const wcsph_sim* s) {
     a.h = s->hmax;
     auto splat = [](double v) { return h2{(_Float16)(float)v, (_Float16)(float)v}; };
     a.k_eps = splat(s->p.eps);
-    a.malpha = (float)(-2.0 * s->p.alpha);
     a.beta = (float)(2.0 * s->p.beta);
     a.k_Ls = splat(s->grid.L / (double)s->hmax);
     a.inv_rho0 = 1.0 / s->p.rho0;
