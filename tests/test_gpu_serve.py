@@ -96,12 +96,15 @@ def test_serving_footprint_is_exact(W, precision):
         W.serve([r], max_slots=1, hbm_budget_bytes=b - 1)
 
 
+@pytest.mark.parametrize("wall_mode", [0, 1])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_grouped_steps_equal_simulation(W, dim):
+def test_grouped_steps_equal_simulation(W, dim, wall_mode):
     # the grouped step kernels (serving) give the results of the ordinary
     # step kernels (a Simulation stepped directly), bit for bit, also when
-    # the step count is odd (the last step runs as a one-member launch)
+    # the step count is odd (the last step runs as a one-member launch),
+    # with frozen walls (wall_mode 1) as without
     r = W.make_request(dim, 30_000, 1.5, 0.1, 121, 0.0, "mixed")
+    r.params.wall_mode = wall_mode
     res = W.serve([r], max_slots=1, chunk_steps=40)[0]
     assert res["status"] == 0 and res["precision_used"] == W.PREC_MIXED
     st = W.dam_break_state(r.geom, r.params)
