@@ -48,8 +48,7 @@ __device__ __forceinline__ int group_member(const GroupMap& gm) {
 // same fields, so both run with identical arguments).
 struct ScalarsArgs {
     Scalars* s;
-    const double* dtmin;
-    int64_t nblk;
+    Partials P;  // the member's dt bounds and pending pair counts
     int32_t mode;
 };
 
@@ -108,7 +107,7 @@ struct ReorderArgs {
 
 static __global__ __launch_bounds__(kScalarsThreads) void kg_scalars(ctab<ScalarsArgs> t) {
     const ScalarsArgs& a = args_at(t + blockIdx.x);
-    scalars_body(a.s, a.dtmin, a.nblk, a.mode);
+    scalars_body(a.s, a.P, a.mode);
 }
 
 template <int D>

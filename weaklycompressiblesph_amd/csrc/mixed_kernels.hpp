@@ -534,7 +534,7 @@ __device__ __forceinline__ void nforce_mixed_body(VBlock vb, const NFMixedArgs& 
 #pragma unroll
         for (int d = 0; d < D; ++d) a.acc[d * n + i] = ad[d];
     }
-    block_count_atomic(total, a.s, true, a.fused_final ? &a.P.dtmin[vb.id] : nullptr, cand);
+    block_count_store(total, a.P, vb.id, a.fused_final != 0, cand);
 }
 
 template <int D, int R, int VS, int BT>

@@ -147,10 +147,12 @@ void run_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const PassG
     uint32_t* tile_total = at<uint32_t>(ws, L.tile_total);
     uint32_t* group_total = at<uint32_t>(ws, L.group_total);
     Scalars* sc = at<Scalars>(ws, L.sc);
+    const Partials P{at<unsigned long long>(ws, L.bpairs), at<uint32_t>(ws, L.bmax), nullptr, nullptr,
+                     ceil_div(n, kBlock), at<uint32_t>(ws, L.pending)};
     Packed pk{at<float4>(ws, L.px), at<float4>(ws, L.pv), at<float4>(ws, L.pq), at<uint32_t>(ws, L.pcell)};
     const int64_t npart = ceil_div(ncell + 1, kScanTile);
     const dim3 blk(kBlock), grid_n((unsigned)ceil_div(n, kBlock));
-    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, st, sc, (const double*)nullptr, (int64_t)0, 0);
+    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, st, sc, P, 0);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)ceil_div(ncell + 1, kBlock)), blk, 0, st, count, ncell + 1, 0u);
     const int64_t gsize = scan_group_size(npart), ngroups = ceil_div(npart, gsize);
     Master m{};
@@ -192,10 +194,14 @@ void run_force_pass(const wcsph_params* p, const wcsph_pass_desc* d, const PassG
     a.dti = dt_i;
     a.s = sc;
     a.fused_final = 0;
+    a.P = P;
     a.out_index = id;
     if (D == 3) p->vsig_mode ? launch_pass_force<3, 1>(a, st) : launch_pass_force<3, 0>(a, st);
     else p->vsig_mode ? launch_pass_force<2, 1>(a, st) : launch_pass_force<2, 0>(a, st);
-    if (pairs) hipLaunchKernelGGL(k_pass_pairs, dim3(1), dim3(64), 0, st, (const Scalars*)sc, pairs);
+    if (pairs) {
+        hipLaunchKernelGGL(k_fold_counts, dim3(1), dim3(kBlock), 0, st, P, sc);
+        hipLaunchKernelGGL(k_pass_pairs, dim3(1), dim3(64), 0, st, (const Scalars*)sc, pairs);
+    }
     WCSPH_HIP(hipGetLastError());
 }
 

@@ -12,7 +12,8 @@ namespace wcsph {
 
 // Byte offsets of the pass's workspace arrays (256-byte aligned).
 struct PassLayout {
-    size_t key, slot, perm, id, px, pv, pq, pcell, count, start, tile_total, group_total, sc, total;
+    size_t key, slot, perm, id, px, pv, pq, pcell, count, start, tile_total, group_total, bpairs, bmax, pending, sc,
+        total;
 };
 
 // The cell grid of a pass and its size, from the caller's description
@@ -44,6 +45,10 @@ constexpr PassLayout pass_layout(int64_t n, int64_t ncell) {
     L.start = take(4 * (size_t)(ncell + 2));
     L.tile_total = take(4 * (size_t)((ncell + 1 + kScanTile - 1) / kScanTile));
     L.group_total = take(4 * (size_t)kScanMaxGroups);
+    // the force kernel's per-block pair counts and their pending flag
+    L.bpairs = take(8 * (size_t)((n + kBlock - 1) / kBlock));
+    L.bmax = take(4 * (size_t)((n + kBlock - 1) / kBlock));
+    L.pending = take(4);
     L.sc = take(sizeof(Scalars));
     L.total = off;
     return L;

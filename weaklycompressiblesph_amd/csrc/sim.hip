@@ -193,6 +193,7 @@ int64_t sim_footprint(int D, int64_t n, int precision, int K, int64_t ncell) {
     add(4 * nb);  // maxcnt
     add(4 * nb);  // over
     add(8 * nb);  // dtmin
+    add(4);       // pending (the force pass's counts not yet folded)
     add(8 * nb * kDiag);
     add(8 * kDiag);
     // cell grid: counts and starts with the dead bucket, scan totals
@@ -220,6 +221,12 @@ int64_t sim_grid_cells(const wcsph_params* p, const wcsph_sim_config* cfg, int64
 }
 
 }  // namespace wcsph
+
+// Fold a pending force pass's per-block pair counts into the step scalars,
+// behind the queued work (before the host reads them).
+static void fold_counts_now(wcsph_sim* s) {
+    hipLaunchKernelGGL(k_fold_counts, dim3(1), dim3(kBlock), 0, s->st, s->P, s->sc);
+}
 
 // Copy the device failure flags to the pinned mirror behind the queued work.
 static void queue_flags(wcsph_sim* s) {
@@ -317,7 +324,7 @@ static void setup_precision(wcsph_sim* s) {
 }
 
 // Argument records of the step kernels for the current state.
-static ScalarsArgs scalars_args(const wcsph_sim* s, int mode) { return ScalarsArgs{s->sc, s->P.dtmin, s->P.nblk, mode}; }
+static ScalarsArgs scalars_args(const wcsph_sim* s, int mode) { return ScalarsArgs{s->sc, s->P, mode}; }
 
 template <int D>
 static BinArgs<D> bin_args(const wcsph_sim* s, int kick) {
@@ -688,7 +695,7 @@ static bool shepard_due(const wcsph_sim* s, int64_t step) {
 static void one_step(wcsph_sim* s, bool filter) {
     hipEvent_t* ev = s->timing ? step_events(s) : nullptr;
     if (ev) WCSPH_HIP(hipEventRecord(ev[0], s->st));
-    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 1);
+    hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P, 1);
     bin_and_reorder(s, true);
     if (filter) shepard_pass(s);  // after the drift, before the pressure (src/main.cpp:105-106)
     // the fused force passes end with the final stage; the FP64 path runs it
@@ -864,6 +871,8 @@ wcsph_sim* wcsph::sim_create_in(const wcsph_params* p, const wcsph_sim_config* c
         s->P.maxcnt = s->alloc<uint32_t>(s->P.nblk);
         s->P.over = s->alloc<uint32_t>(s->P.nblk);
         s->P.dtmin = s->alloc<double>(s->P.nblk);
+        s->P.pending = s->alloc<uint32_t>(1);
+        WCSPH_HIP(hipMemsetAsync(s->P.pending, 0, sizeof(uint32_t), s->st));
         s->diag_part = s->alloc<double>(s->nb(n) * kDiag);
         s->diag_out = s->alloc<double>(kDiag);
         WCSPH_HIP(hipMemsetAsync(s->acc, 0, sizeof(double) * D * n, s->st));
@@ -938,7 +947,7 @@ int wcsph_sim_init(wcsph_sim* s) {
     return guarded([&] {
         WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "upload a state first");
         WCSPH_HIP(hipSetDevice(s->device));
-        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 0);
+        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P, 0);
         bin_and_reorder(s, false);
         neigh_force(s, nullptr, nullptr, nullptr);
         if (s->p.shepard_interval > 0) shepard_pass(s);  // after the first force (src/main.cpp:75)
@@ -1022,7 +1031,7 @@ int wcsph_sim_force(wcsph_sim* s) {
     return guarded([&] {
         WCSPH_REQUIRE(s && s->uploaded, WCSPH_ESTATE, "upload a state first");
         WCSPH_HIP(hipSetDevice(s->device));
-        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
+        hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P, 2);
         s->steppable = false;  // acc is hydro only and the dt bounds are stale
         WCSPH_HIP(hipEventRecord(s->ev[0], s->st));
         bin_and_reorder(s, false);
@@ -1093,6 +1102,8 @@ int wcsph_sim_sync(wcsph_sim* s) {
 int wcsph_sim_get_stats(wcsph_sim* s, wcsph_sim_stats* st) {
     return guarded([&] {
         WCSPH_REQUIRE(s && st, WCSPH_EINVAL, "null argument");
+        WCSPH_HIP(hipSetDevice(s->device));
+        fold_counts_now(s);
         Scalars h{};
         WCSPH_HIP(hipMemcpyAsync(&h, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost, s->st));
         WCSPH_HIP(hipStreamSynchronize(s->st));
@@ -1144,7 +1155,7 @@ int wcsph_sim_neighbours(wcsph_sim* s, int32_t* counts, int32_t* lists, int32_t 
             // the fused pass keeps no lists: rebuild them for the current cell
             // order (the mixed path keeps no FP32 records either: write them)
             if (s->cfg.precision == WCSPH_PREC_MIXED) fp32_records(s);
-            hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P.dtmin, s->P.nblk, 2);
+            hipLaunchKernelGGL(k_scalars, dim3(1), dim3(kScalarsThreads), 0, s->st, s->sc, s->P, 2);
             neigh_lists(s, 0);
             WCSPH_HIP(hipGetLastError());
         }
@@ -1358,6 +1369,8 @@ int wcsph_sim_save(wcsph_sim* s, const char* path) {
         raise_if_failed(s);  // a diverged state is not worth resuming
         HostState hs;
         fetch_state(s, hs);
+        fold_counts_now(s);  // the header carries the pair counts
+        WCSPH_HIP(hipStreamSynchronize(s->st));
         Scalars sc{};
         WCSPH_HIP(hipMemcpy(&sc, s->sc, sizeof(Scalars), hipMemcpyDeviceToHost));
         std::vector<double> dtm((size_t)s->P.nblk);

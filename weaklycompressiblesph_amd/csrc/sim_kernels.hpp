@@ -79,14 +79,56 @@ __device__ __forceinline__ uint32_t dead_key(const Grid& g) {
 }
 
 // Per-block partial results (one slot per block of the particle kernels),
-// reduced in index order by k_reduce_partials: no same-address atomics.
+// reduced by a one-block pass: no same-address atomics, which serialise
+// far from the cores across the eight XCDs (one per block on one address
+// cost a 2D force pass of 2,770 blocks more than half its time). The
+// neighbour-list pass's counts are reduced at once by k_reduce_partials;
+// a force pass's counts stay pending (*pending = 1) until the next
+// k_scalars or k_fold_counts folds them into the step scalars.
 struct Partials {
     unsigned long long* pairs;  // pair count per block
     uint32_t* maxcnt;           // largest per-particle count per block
     uint32_t* over;             // list overflow flag per block
     double* dtmin;              // time-step bound per block
     int64_t nblk;
+    uint32_t* pending;          // 1: pairs/maxcnt hold a force pass not yet folded
 };
+
+// Fold a pending force pass's per-block counts into the step scalars:
+// pairs_last and maxcnt of that pass, pairs_total += its pairs. Every thread
+// of the block calls it (blockDim.x a multiple of 64, at most 1024).
+__device__ __forceinline__ void fold_counts(const Partials& P, Scalars* s) {
+    __shared__ unsigned long long fsum[16];
+    __shared__ uint32_t fmax[16];
+    if (P.pending == nullptr || *P.pending == 0u) return;  // the same value for every thread
+    unsigned long long v = 0;
+    uint32_t m = 0;
+    for (int64_t b = threadIdx.x; b < P.nblk; b += blockDim.x) {
+        v += P.pairs[b];
+        m = P.maxcnt[b] > m ? P.maxcnt[b] : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_down(v, off, 64);
+        const uint32_t o = __shfl_down(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        fsum[threadIdx.x >> 6] = v;
+        fmax[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            v += fsum[k];
+            m = fmax[k] > m ? fmax[k] : m;
+        }
+        s->pairs_last = v;
+        s->maxcnt = m;
+        s->pairs_total += v;
+        *P.pending = 0u;
+    }
+    __syncthreads();
+}
 
 // Map blockIdx onto particle blocks for the 8 XCDs (hardware places block b
 // on XCD b % 8, each XCD with its own L2). WCSPH_XCD_CHUNK > 0: each XCD
@@ -143,36 +185,78 @@ __device__ __forceinline__ uint32_t cell_key(const Grid& g, const int* c) {
 
 // mode 0: full reset; 1: commit dt = the minimum of the last final stage's
 // per-block bounds, time += dt, step += 1; 2: reset the per-pass counters
-// only. One block of kScalarsThreads threads, four loads in flight each.
+// only. Modes 1 and 2 first fold a pending force pass's counts; mode 0
+// drops them. One block of kScalarsThreads threads, four loads in flight
+// each.
 constexpr int kScalarsThreads = 1024;
-__device__ __forceinline__ void scalars_body(Scalars* s, const double* __restrict__ dtmin, int64_t nblk, int mode) {
+__device__ __forceinline__ void scalars_body(Scalars* s, const Partials& P, int mode) {
     __shared__ double sh[kScalarsThreads / 64];
-    if (mode == 1) {
+    __shared__ unsigned long long shp[kScalarsThreads / 64];
+    __shared__ uint32_t shm[kScalarsThreads / 64];
+    const int64_t nblk = P.nblk;
+    // a pending force pass's pair counts are folded in the same pass over
+    // the per-block slots as the dt bounds (the flag reads the same in every
+    // thread: only thread 0 clears it, after the reduction)
+    const bool fold = mode != 0 && P.pending != nullptr && *P.pending != 0u;
+    const bool dt = mode == 1;
+    double d = 1.0e30;
+    unsigned long long v = 0;
+    uint32_t m = 0;
+    if (fold || dt) {
         const int64_t T = kScalarsThreads;
         double d0 = 1.0e30, d1 = 1.0e30, d2 = 1.0e30, d3 = 1.0e30;
         int64_t b = threadIdx.x;
         for (; b + 3 * T < nblk; b += 4 * T) {
-            const double x0 = dtmin[b], x1 = dtmin[b + T], x2 = dtmin[b + 2 * T], x3 = dtmin[b + 3 * T];
-            d0 = x0 < d0 ? x0 : d0;
-            d1 = x1 < d1 ? x1 : d1;
-            d2 = x2 < d2 ? x2 : d2;
-            d3 = x3 < d3 ? x3 : d3;
+            if (dt) {
+                const double x0 = P.dtmin[b], x1 = P.dtmin[b + T], x2 = P.dtmin[b + 2 * T], x3 = P.dtmin[b + 3 * T];
+                d0 = x0 < d0 ? x0 : d0;
+                d1 = x1 < d1 ? x1 : d1;
+                d2 = x2 < d2 ? x2 : d2;
+                d3 = x3 < d3 ? x3 : d3;
+            }
+            if (fold) {
+                v += (P.pairs[b] + P.pairs[b + T]) + (P.pairs[b + 2 * T] + P.pairs[b + 3 * T]);
+                const uint32_t m0 = max(P.maxcnt[b], P.maxcnt[b + T]), m1 = max(P.maxcnt[b + 2 * T], P.maxcnt[b + 3 * T]);
+                m = max(m, max(m0, m1));
+            }
         }
-        for (; b < nblk; b += T) d0 = dtmin[b] < d0 ? dtmin[b] : d0;
+        for (; b < nblk; b += T) {
+            if (dt) d0 = P.dtmin[b] < d0 ? P.dtmin[b] : d0;
+            if (fold) {
+                v += P.pairs[b];
+                m = max(m, P.maxcnt[b]);
+            }
+        }
         d0 = d1 < d0 ? d1 : d0;
         d2 = d3 < d2 ? d3 : d2;
-        double d = d2 < d0 ? d2 : d0;
+        d = d2 < d0 ? d2 : d0;
         for (int off = 32; off > 0; off >>= 1) {
             const double o = __shfl_down(d, off, 64);
             d = o < d ? o : d;
+            v += __shfl_down(v, off, 64);
+            const uint32_t om = __shfl_down(m, off, 64);
+            m = om > m ? om : m;
         }
-        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = d;
+        if ((threadIdx.x & 63) == 0) {
+            sh[threadIdx.x >> 6] = d;
+            shp[threadIdx.x >> 6] = v;
+            shm[threadIdx.x >> 6] = m;
+        }
         __syncthreads();
         if (threadIdx.x == 0)
-            for (int k = 1; k < kScalarsThreads / 64; ++k) sh[0] = sh[k] < sh[0] ? sh[k] : sh[0];
+            for (int k = 1; k < kScalarsThreads / 64; ++k) {
+                d = sh[k] < d ? sh[k] : d;
+                v += shp[k];
+                m = shm[k] > m ? shm[k] : m;
+            }
     }
     if (threadIdx.x != 0) return;
+    if (fold) {  // the folded pass's pairs_last and maxcnt are replaced by the new pass's
+        s->pairs_total += v;
+        *P.pending = 0u;
+    }
     if (mode == 0) {
+        if (P.pending) *P.pending = 0u;
         s->dt = 0.0;
         s->time = 0.0;
         s->step = 0;
@@ -182,8 +266,8 @@ __device__ __forceinline__ void scalars_body(Scalars* s, const double* __restric
         s->overfull = 0;
     }
     if (mode == 1) {
-        s->dt = sh[0];
-        s->dtmin_bits = (unsigned long long)__double_as_longlong(sh[0]);
+        s->dt = d;
+        s->dtmin_bits = (unsigned long long)__double_as_longlong(d);
         s->time += s->dt;
         s->step += 1;
     }
@@ -192,10 +276,12 @@ __device__ __forceinline__ void scalars_body(Scalars* s, const double* __restric
     s->maxcnt = 0;
 }
 
-static __global__ __launch_bounds__(kScalarsThreads) void k_scalars(Scalars* s, const double* __restrict__ dtmin,
-                                                              int64_t nblk, int mode) {
-    scalars_body(s, dtmin, nblk, mode);
+static __global__ __launch_bounds__(kScalarsThreads) void k_scalars(Scalars* s, Partials P, int mode) {
+    scalars_body(s, P, mode);
 }
+
+// The fold alone (before the host reads the step scalars).
+static __global__ __launch_bounds__(kBlock) void k_fold_counts(Partials P, Scalars* s) { fold_counts(P, s); }
 
 // Counting sort, pass 1: cell key and slot within the cell, after the
 // first half of a step when kick != 0: initialKick and fullDrift
@@ -725,14 +811,13 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_partials(Partials P, S
     reduce_partials_block(P, s, what, sh);
 }
 
-// Block totals of the fused force passes, added straight into the step
-// scalars with one integer atomic each per block: integer sums, maxima and
-// ors do not depend on the order the blocks arrive in.
-__device__ __forceinline__ void block_count_atomic(uint32_t cnt, Scalars* s, bool to_total,
-                                                   double* dt_slot = nullptr, double dt_cand = 1.0e30) {
-    // one barrier for both block reductions: the pair count (sum and max,
-    // added to the scalars with one atomic each) and, with dt_slot, the
-    // minimum of the positive dt bounds (they order like their bit patterns)
+// One barrier for the block's reductions of a force pass: the pair count
+// (sum and max) into the block's partial slots, and, with with_dt, the
+// minimum of the positive dt bounds (they order like their bit patterns)
+// into its dt slot. Block 0 marks the pass pending (k_scalars or
+// k_fold_counts folds it). Every thread of the block calls it.
+__device__ __forceinline__ void block_count_store(uint32_t cnt, const Partials& P, int64_t blk, bool with_dt,
+                                                  double dt_cand) {
     __shared__ unsigned long long sh_sum[kBlock / 64];
     __shared__ uint32_t sh_max[kBlock / 64];
     __shared__ unsigned long long sh_dt[kBlock / 64];
@@ -743,7 +828,7 @@ __device__ __forceinline__ void block_count_atomic(uint32_t cnt, Scalars* s, boo
         v += __shfl_down(v, off, 64);
         const uint32_t o = __shfl_down(mx, off, 64);
         mx = o > mx ? o : mx;
-        if (dt_slot) {
+        if (with_dt) {
             const unsigned long long ob = __shfl_down(b, off, 64);
             b = ob < b ? ob : b;
         }
@@ -763,10 +848,10 @@ __device__ __forceinline__ void block_count_atomic(uint32_t cnt, Scalars* s, boo
             m = sh_max[k] > m ? sh_max[k] : m;
             tb = sh_dt[k] < tb ? sh_dt[k] : tb;
         }
-        if (dt_slot) *dt_slot = __longlong_as_double((long long)tb);
-        atomicAdd(&s->pairs_last, t);
-        if (to_total) atomicAdd(&s->pairs_total, t);
-        atomicMax(&s->maxcnt, m);
+        if (with_dt) P.dtmin[blk] = __longlong_as_double((long long)tb);
+        P.pairs[blk] = t;
+        P.maxcnt[blk] = m;
+        if (blk == 0) *P.pending = 1u;
     }
 }
 
@@ -1189,7 +1274,7 @@ __global__ __launch_bounds__(kBlock) void k_nforce(NFArgs a) {
 #pragma unroll
         for (int d = 0; d < D; ++d) a.acc[d * n + o] = ad[d];
     }
-    block_count_atomic(total, a.s, true, a.fused_final ? &a.P.dtmin[blockIdx.x] : nullptr, cand);
+    block_count_store(total, a.P, blockIdx.x, a.fused_final != 0, cand);
 }
 
 // Shepard density filter (ApplyFilter, src/force.h:10-30, with the Gather
