@@ -300,7 +300,6 @@ __device__ __forceinline__ void bin_body(int64_t blk, Master m, const double* __
     const int64_t i = blk * blockDim.x + threadIdx.x;
     const bool live = i < n;
     uint32_t k = 0xffffffffu;
-    bool escaped = false, nonfinite = false;  // counted once per wave below
     if (live) {
         double x[D];
 #pragma unroll
@@ -326,19 +325,14 @@ __device__ __forceinline__ void bin_body(int64_t blk, Master m, const double* __
             bool esc = false;
             cell_of<D>(g, x, c, esc);
             k = cell_key<D>(g, c);
-            escaped = esc;
+            if (esc) atomicAdd(&s->escaped, 1u);
         } else {
             k = dead_key<D>(g);
-            nonfinite = true;
+            atomicAdd(&s->nonfinite, 1u);
         }
         key[i] = k;
     }
     const int lane = threadIdx.x & 63;
-    // the escaped and non-finite counts, one atomic per wave (not per particle:
-    // same-address atomics serialise across the XCDs)
-    const unsigned long long em = __ballot(escaped), nm = __ballot(nonfinite);
-    if (lane == 0 && em) atomicAdd(&s->escaped, (uint32_t)__popcll(em));
-    if (lane == 0 && nm) atomicAdd(&s->nonfinite, (uint32_t)__popcll(nm));
     const uint32_t kp = (uint32_t)__shfl_up((int)k, 1, 64);
     const bool lead = lane == 0 || kp != k;  // first lane of a run (dead lanes form their own run)
     const unsigned long long leads = __ballot(lead);
